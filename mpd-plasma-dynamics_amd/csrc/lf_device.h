@@ -1,0 +1,204 @@
+// lf_device.h -- device-side math of the Lax-Friedrichs step shared by the
+// gfx950 kernels.  Every function restates a reference formula in the
+// reference's exact operation order (IEEE binary64, no contraction: the
+// library is compiled with -ffp-contract=off).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lf_types.h"
+
+namespace mpd {
+
+// Kernel parameters of one step (all device pointers).
+struct StepArgs {
+    int L;          // L_max: interior rows 1..L
+    int Mn;         // M_max + 1: interior columns 1..Mn
+    int pitch;      // row pitch in doubles
+    int64_t fstride;  // distance between two fields of one state buffer, in doubles
+    const double* u_in;   // u0 (start of step), 8 fields
+    double* u_out;        // u (end of step), 8 fields
+    double* rowbuf;       // central u of rows 1 and L: [2][8][pitch]
+    const double* phys_in;  // step-1 physical fields (10, same layout) or nullptr
+    const double* rin;    // per row l = 0..L+1: r_inner(z_l)
+    const double* rout;   // r_outer(z_l)
+    const double* drin;   // d r_inner / dz
+    const double* drout;  // d r_outer / dz
+    const double* dr;     // dr[l] = (r_outer - r_inner) / M_max
+    double gamma, gm1, beta, H_z0, r_0, dz, dy, dz2;
+    double cfl_dx, growth, dt_min, dt_max, T;
+    int adaptive;
+    int step1;            // start-of-step physical comes from phys_in
+    MpdLoopState* st;
+    const MpdBcProgram* bc;
+};
+
+// Physical fields in the member order of include/fields.hpp:36-41.
+enum { PH_RHO = 0, PH_VZ, PH_VR, PH_VPHI, PH_E, PH_P, PH_PT, PH_HZ, PH_HR, PH_HPHI };
+
+// Per-cell record used by the stencil: the 8 conserved values, the physical
+// values its neighbours read, and r.
+struct Rec {
+    double u[8];
+    double vz, vr, vphi, P, Hz, Hr, Hphi, r;
+    double rho, p;
+};
+
+__device__ __forceinline__ double cell_r(const StepArgs& a, int l, int m) {
+    // Grid::Build, src/grid.cpp:31-37: frac = m_global * dy, m_global = m - 1.
+    const double frac = (double)(m - 1) * a.dy;
+    return (1.0 - frac) * a.rin[l] + frac * a.rout[l];
+}
+
+__device__ __forceinline__ double cell_rz(const StepArgs& a, int l, int m) {
+    const double frac = (double)(m - 1) * a.dy;
+    return (1.0 - frac) * a.drin[l] + frac * a.drout[l];
+}
+
+// Fields::UpdatePhysicalFromU, src/fields.cpp:84-97.
+__device__ __forceinline__ void rec_phys(Rec& c, double gm1) {
+    c.rho = c.u[0] / c.r;
+    c.vz = c.u[1] / c.u[0];
+    c.vr = c.u[2] / c.u[0];
+    c.vphi = c.u[3] / c.u[0];
+    c.Hphi = c.u[5];
+    c.Hz = c.u[6] / c.r;
+    c.Hr = c.u[7] / c.r;
+    const double e = c.u[4] / c.u[0];
+    c.p = gm1 * c.rho * e;
+    c.P = c.p + 0.5 * (c.Hz * c.Hz + c.Hr * c.Hr + c.Hphi * c.Hphi);
+}
+
+__device__ __forceinline__ void phys10_from_u(const double* u, double r, double gm1, double* ph) {
+    ph[PH_RHO] = u[0] / r;
+    ph[PH_VZ] = u[1] / u[0];
+    ph[PH_VR] = u[2] / u[0];
+    ph[PH_VPHI] = u[3] / u[0];
+    ph[PH_HPHI] = u[5];
+    ph[PH_HZ] = u[6] / r;
+    ph[PH_HR] = u[7] / r;
+    ph[PH_E] = u[4] / u[0];
+    ph[PH_P] = gm1 * ph[PH_RHO] * ph[PH_E];
+    ph[PH_PT] = ph[PH_P] + 0.5 * (ph[PH_HZ] * ph[PH_HZ] + ph[PH_HR] * ph[PH_HR] + ph[PH_HPHI] * ph[PH_HPHI]);
+}
+
+// Diagnostics::MaxWaveSpeed's per-cell term, src/diagnostics.cpp:26-36, on
+// the physical values f(u) of conserved values u.
+__device__ __forceinline__ double wave_speed_u(const double* u, double r, double gamma, double gm1) {
+    const double rho = u[0] / r;
+    const double vz = u[1] / u[0];
+    const double vr = u[2] / u[0];
+    const double hphi = u[5];
+    const double hz = u[6] / r;
+    const double hr = u[7] / r;
+    const double e = u[4] / u[0];
+    const double p = gm1 * rho * e;
+    const double cs = sqrt(gamma * p / rho);
+    const double ca = sqrt((hz * hz + hr * hr + hphi * hphi) / rho);
+    const double v = sqrt(vz * vz + vr * vr);
+    return v + cs + ca;
+}
+
+__device__ __forceinline__ double wave_speed_ph(const double* ph, double gamma) {
+    const double rho = ph[PH_RHO];
+    const double hz = ph[PH_HZ], hr = ph[PH_HR], hphi = ph[PH_HPHI];
+    const double vz = ph[PH_VZ], vr = ph[PH_VR];
+    const double cs = sqrt(gamma * ph[PH_P] / rho);
+    const double ca = sqrt((hz * hz + hr * hr + hphi * hphi) / rho);
+    const double v = sqrt(vz * vz + vr * vr);
+    return v + cs + ca;
+}
+
+// Correctly rounded x / c for a positive normal constant c with cinv =
+// RN(1/c): a product estimate refined by two residual corrections; the last
+// correction starts within one ulp, the case in which Markstein's theorem
+// makes RN(q + r * cinv) with the exact residual r = x - q * c equal to
+// RN(x / c) (Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al.,
+// Handbook of Floating-Point Arithmetic, 2nd ed., sec. 4.9).  The sign of a
+// zero quotient is restored from x.  Valid for normal, finite x.
+__device__ __forceinline__ double div_const(double x, double c, double cinv) {
+    const double q0 = x * cinv;
+    const double r0 = __builtin_fma(-q0, c, x);
+    const double q1 = __builtin_fma(r0, cinv, q0);
+    const double r1 = __builtin_fma(-q1, c, x);
+    const double q2 = __builtin_fma(r1, cinv, q1);
+    return __builtin_copysign(q2, x);
+}
+
+struct DivExact {
+    __device__ __forceinline__ double operator()(double x, double c, double) const { return x / c; }
+};
+struct DivConst {
+    __device__ __forceinline__ double operator()(double x, double c, double ci) const {
+        return div_const(x, c, ci);
+    }
+};
+
+// Solver::ComputeCentralUpdate's cell update, src/solver.cpp:94-150.
+// N/S: l+1 / l-1; E/W: m+1 / m-1; C: the cell; dz2 = 2*dz, dr2 = 2*dr_l.
+template <class Div>
+__device__ __forceinline__ void lf_cell(const Rec& N, const Rec& S, const Rec& E, const Rec& W,
+                                        const Rec& C, double dt, double dz2, double dz2i,
+                                        double dr2, double dr2i, double* o) {
+    const Div D;
+    o[0] = 0.25 * (N.u[0] + S.u[0] + E.u[0] + W.u[0]) +
+           dt * (D(-(N.u[0] * N.vz - S.u[0] * S.vz), dz2, dz2i) -
+                 D(E.u[0] * E.vr - W.u[0] * W.vr, dr2, dr2i));
+
+    o[1] = 0.25 * (N.u[1] + S.u[1] + E.u[1] + W.u[1]) +
+           dt * (D((N.Hz * N.Hz - N.P) * N.r - (S.Hz * S.Hz - S.P) * S.r, dz2, dz2i) +
+                 D(E.Hz * E.Hr * E.r - W.Hz * W.Hr * W.r, dr2, dr2i) -
+                 D(N.u[1] * N.vz - S.u[1] * S.vz, dz2, dz2i) -
+                 D(E.u[1] * E.vr - W.u[1] * W.vr, dr2, dr2i));
+
+    o[2] = 0.25 * (N.u[2] + S.u[2] + E.u[2] + W.u[2]) +
+           dt * ((C.rho * C.vphi * C.vphi + C.P - C.Hphi * C.Hphi) +
+                 D(N.Hz * N.Hr * N.r - S.Hz * S.Hr * S.r, dz2, dz2i) +
+                 D((E.Hr * E.Hr - E.P) * E.r - (W.Hr * W.Hr - W.P) * W.r, dr2, dr2i) -
+                 D(N.u[2] * N.vz - S.u[2] * S.vz, dz2, dz2i) -
+                 D(E.u[2] * E.vr - W.u[2] * W.vr, dr2, dr2i));
+
+    o[3] = 0.25 * (N.u[3] + S.u[3] + E.u[3] + W.u[3]) +
+           dt * ((-C.rho * C.vr * C.vphi + C.Hphi * C.Hr) +
+                 D(N.Hphi * N.Hz * N.r - S.Hphi * S.Hz * S.r, dz2, dz2i) +
+                 D(E.Hphi * E.Hr * E.r - W.Hphi * W.Hr * W.r, dr2, dr2i) -
+                 D(N.u[3] * N.vz - S.u[3] * S.vz, dz2, dz2i) -
+                 D(E.u[3] * E.vr - W.u[3] * W.vr, dr2, dr2i));
+
+    o[4] = 0.25 * (N.u[4] + S.u[4] + E.u[4] + W.u[4]) +
+           dt * (-C.p * (D(N.vz * N.r - S.vz * S.r, dz2, dz2i) +
+                         D(E.vr * E.r - W.vr * W.r, dr2, dr2i)) -
+                 D(N.u[4] * N.vz - S.u[4] * S.vz, dz2, dz2i) -
+                 D(E.u[4] * E.vr - W.u[4] * W.vr, dr2, dr2i));
+
+    o[5] = 0.25 * (N.u[5] + S.u[5] + E.u[5] + W.u[5]) +
+           dt * (D(N.Hz * N.vphi - S.Hz * S.vphi, dz2, dz2i) +
+                 D(E.Hr * E.vphi - W.Hr * W.vphi, dr2, dr2i) -
+                 D(N.u[5] * N.vz - S.u[5] * S.vz, dz2, dz2i) -
+                 D(E.u[5] * E.vr - W.u[5] * W.vr, dr2, dr2i));
+
+    o[6] = 0.25 * (N.u[6] + S.u[6] + E.u[6] + W.u[6]) +
+           dt * (D(E.Hr * E.vz * E.r - W.Hr * W.vz * W.r, dr2, dr2i) -
+                 D(E.u[6] * E.vr - W.u[6] * W.vr, dr2, dr2i));
+
+    o[7] = 0.25 * (N.u[7] + S.u[7] + E.u[7] + W.u[7]) +
+           dt * (D(N.Hz * N.vr * N.r - S.Hz * S.vr * S.r, dz2, dz2i) -
+                 D(N.u[7] * N.vz - S.u[7] * S.vz, dz2, dz2i));
+}
+
+// Max of non-negative doubles through their bit patterns; NaN is skipped as
+// std::max(local_max, v) skips it in src/diagnostics.cpp:36.
+__device__ __forceinline__ uint64_t speed_key(double s) {
+    return (s == s) ? (uint64_t)__double_as_longlong(s) : 0ull;
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(v, off, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+}  // namespace mpd

@@ -1,0 +1,680 @@
+// lf_kernels.hip -- gfx950 kernels of the explicit Lax-Friedrichs step.
+//
+//   lf_stencil_*      the fused flux-plus-update stencil over the 8 conserved
+//                     variables with the axisymmetric sources
+//                     (Solver::ComputeCentralUpdate, src/solver.cpp:59-153),
+//                     physical fields recomputed in registers from u0, and the
+//                     max wave speed of the new state as an epilogue
+//                     (Diagnostics::MaxWaveSpeed, src/diagnostics.cpp:14-43).
+//   lf_bc_ring        the four faces L_LO, M_HI, M_LO, L_HI in the reference
+//                     order (src/solver.cpp:29-32, src/face_bc.cpp,
+//                     src/bcs/per_field_bc.cpp), one thread per boundary cell,
+//                     plus the dt update of the time loop (src/main.cpp:221-231).
+//   lf_residual       SolutionChange + SavePrev (src/diagnostics.cpp:71-106,
+//                     src/fields.cpp:59-74).
+//   reductions        max wave speed, conserved integrals, outlet integrals,
+//                     radial-symmetry metric, physical-field download.
+#include <hip/hip_runtime.h>
+
+#include "lf_device.h"
+
+namespace mpd {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ const double* fld(const double* base, int64_t fstride, int k) {
+    return base + (int64_t)k * fstride;
+}
+
+__device__ __forceinline__ int64_t at(const StepArgs& a, int l, int m) {
+    return (int64_t)l * a.pitch + m;
+}
+
+// Start-of-step physical value set at an interior cell: f(u0), or the IC
+// arrays on the first step (src/main.cpp:144-152 sets them before step 1).
+__device__ __forceinline__ void start_phys(const StepArgs& a, int l, int m, double* ph) {
+    if (a.step1) {
+        for (int k = 0; k < 10; ++k) ph[k] = fld(a.phys_in, a.fstride, k)[at(a, l, m)];
+    } else {
+        double u[8];
+        for (int k = 0; k < 8; ++k) u[k] = fld(a.u_in, a.fstride, k)[at(a, l, m)];
+        phys10_from_u(u, cell_r(a, l, m), a.gm1, ph);
+    }
+}
+
+// The physical state every boundary-condition read starts from (the state
+// after UpdateCentralPhysical, src/solver.cpp:155-159): central cells hold
+// f(u_new), every other interior cell the start-of-step physical values.
+__device__ __forceinline__ void s0_phys(const StepArgs& a, int l, int m, double* ph) {
+    if (m >= 2 && m <= a.Mn - 1) {
+        double u[8];
+        if (l == 1 || l == a.L) {
+            const double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
+            for (int k = 0; k < 8; ++k) u[k] = rb[(int64_t)k * a.pitch + m];
+        } else {
+            for (int k = 0; k < 8; ++k) u[k] = fld(a.u_out, a.fstride, k)[at(a, l, m)];
+        }
+        phys10_from_u(u, cell_r(a, l, m), a.gm1, ph);
+    } else {
+        start_phys(a, l, m, ph);
+    }
+}
+
+// ------------------------------------------------------------- the stencil
+
+// Loads the stencil record of cell (l, m) from u0 (or the IC arrays on the
+// first step).  Ghost rows l = 0 and l = L+1 hold u0 = 0 and physical = 0 in
+// the reference (never written, zero-initialised, src/array2d.cpp:30).
+template <bool kStep1>
+__device__ __forceinline__ void load_rec(const StepArgs& a, int l, int m, Rec& c) {
+    const int64_t i = at(a, l, m);
+    for (int k = 0; k < 8; ++k) c.u[k] = fld(a.u_in, a.fstride, k)[i];
+    c.r = cell_r(a, l, m);
+    if (kStep1) {
+        const double* ph = a.phys_in;
+        c.rho = fld(ph, a.fstride, PH_RHO)[i];
+        c.vz = fld(ph, a.fstride, PH_VZ)[i];
+        c.vr = fld(ph, a.fstride, PH_VR)[i];
+        c.vphi = fld(ph, a.fstride, PH_VPHI)[i];
+        c.p = fld(ph, a.fstride, PH_P)[i];
+        c.P = fld(ph, a.fstride, PH_PT)[i];
+        c.Hz = fld(ph, a.fstride, PH_HZ)[i];
+        c.Hr = fld(ph, a.fstride, PH_HR)[i];
+        c.Hphi = fld(ph, a.fstride, PH_HPHI)[i];
+    } else if (l == 0 || l == a.L + 1) {
+        c.rho = c.vz = c.vr = c.vphi = c.p = c.P = c.Hz = c.Hr = c.Hphi = 0.0;
+    } else {
+        rec_phys(c, a.gm1);
+    }
+}
+
+__device__ __forceinline__ double step_dt(const StepArgs& a, double dt_explicit, int loop) {
+    return loop ? a.st->dt_use : dt_explicit;
+}
+
+// One thread per central cell, every neighbour record loaded and recomputed
+// independently (5x redundant physical recompute): the plain reference-shaped
+// kernel, used for the first step and as a cross-check of the tiled kernel.
+template <bool kStep1, bool kSpeed, class Div>
+__global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_explicit, int loop) {
+    if (loop && a.st->done) return;
+    const int m = 2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int l = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const double dt = step_dt(a, dt_explicit, loop);
+    uint64_t key = 0;
+    if (m <= a.Mn - 1 && l <= a.L) {
+        Rec N, S, E, W, C;
+        load_rec<kStep1>(a, l + 1, m, N);
+        load_rec<kStep1>(a, l - 1, m, S);
+        load_rec<kStep1>(a, l, m + 1, E);
+        load_rec<kStep1>(a, l, m - 1, W);
+        load_rec<kStep1>(a, l, m, C);
+        const double dr_l = a.dr[l];
+        const double dr2 = 2 * dr_l;
+        double o[8];
+        lf_cell<Div>(N, S, E, W, C, dt, a.dz2, 1.0 / a.dz2, dr2, 1.0 / dr2, o);
+        if (l == 1 || l == a.L) {
+            double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
+            for (int k = 0; k < 8; ++k) rb[(int64_t)k * a.pitch + m] = o[k];
+        } else {
+            const int64_t i = at(a, l, m);
+            for (int k = 0; k < 8; ++k) a.u_out[(int64_t)k * a.fstride + i] = o[k];
+            if (kSpeed) key = speed_key(wave_speed_u(o, C.r, a.gamma, a.gm1));
+        }
+    }
+    if (kSpeed) {
+        key = wave_max_u64(key);
+        if ((threadIdx.x & 63) == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
+    }
+}
+
+// ---------------------------------------------------- boundary conditions
+
+// Evaluate one compiled expression (the exprtk call sites of
+// src/bcs/per_field_bc.cpp:323-358) on the symbol values sym[].
+__device__ double eval_expr(const MpdBcProgram* bc, int off, int len, const double* sym) {
+    double st[MPD_MAX_STACK];
+    int sp = 0;
+    for (int i = 0; i < len; ++i) {
+        const MpdInstr in = bc->code[off + i];
+        switch (in.op) {
+            case MPD_OP_CONST: st[sp++] = in.val; break;
+            case MPD_OP_SYM: st[sp++] = sym[in.arg]; break;
+            case MPD_OP_NEG: st[sp - 1] = -st[sp - 1]; break;
+            case MPD_OP_ADD: --sp; st[sp - 1] = st[sp - 1] + st[sp]; break;
+            case MPD_OP_SUB: --sp; st[sp - 1] = st[sp - 1] - st[sp]; break;
+            case MPD_OP_MUL: --sp; st[sp - 1] = st[sp - 1] * st[sp]; break;
+            case MPD_OP_DIV: --sp; st[sp - 1] = st[sp - 1] / st[sp]; break;
+            case MPD_OP_MOD: --sp; st[sp - 1] = fmod(st[sp - 1], st[sp]); break;
+            case MPD_OP_POW: --sp; st[sp - 1] = pow(st[sp - 1], st[sp]); break;
+            case MPD_OP_ATAN2: --sp; st[sp - 1] = atan2(st[sp - 1], st[sp]); break;
+            case MPD_OP_F1: {
+                const double x = st[sp - 1];
+                double y;
+                switch (in.arg) {
+                    case MPD_FN_SIN: y = sin(x); break;
+                    case MPD_FN_COS: y = cos(x); break;
+                    case MPD_FN_TAN: y = tan(x); break;
+                    case MPD_FN_ASIN: y = asin(x); break;
+                    case MPD_FN_ACOS: y = acos(x); break;
+                    case MPD_FN_ATAN: y = atan(x); break;
+                    case MPD_FN_EXP: y = exp(x); break;
+                    case MPD_FN_LOG: y = log(x); break;
+                    case MPD_FN_LOG2: y = log2(x); break;
+                    case MPD_FN_LOG10: y = log10(x); break;
+                    case MPD_FN_SQRT: y = sqrt(x); break;
+                    case MPD_FN_ABS: y = fabs(x); break;
+                    default: y = (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0); break;
+                }
+                st[sp - 1] = y;
+                break;
+            }
+            case MPD_OP_CLAMP: {
+                const double lo = st[sp - 3], x = st[sp - 2], hi = st[sp - 1];
+                sp -= 2;
+                st[sp - 1] = (x < lo) ? lo : ((x > hi) ? hi : x);
+                break;
+            }
+            case MPD_OP_MIN:
+            case MPD_OP_MAX: {
+                const int n = in.arg;
+                double acc = st[sp - n];
+                for (int k = 1; k < n; ++k) {
+                    const double v = st[sp - n + k];
+                    if (in.op == MPD_OP_MIN) acc = (v < acc) ? v : acc;
+                    else acc = (acc < v) ? v : acc;
+                }
+                sp -= n - 1;
+                st[sp - 1] = acc;
+                break;
+            }
+            default: break;
+        }
+    }
+    return st[0];
+}
+
+// Physical slot of each condition (evaluation order rho, v_z, v_r, v_phi,
+// H_z, H_r, H_phi, e) and of each neighbour symbol (SetNeighbors order rho,
+// v_z, v_r, v_phi, e, H_z, H_r, H_phi).
+__constant__ int kCondSlot[8] = {PH_RHO, PH_VZ, PH_VR, PH_VPHI, PH_HZ, PH_HR, PH_HPHI, PH_E};
+__constant__ int kNbSlot[8] = {PH_RHO, PH_VZ, PH_VR, PH_VPHI, PH_E, PH_HZ, PH_HR, PH_HPHI};
+
+__device__ __forceinline__ bool seg_covers(const MpdSeg& s, int i) { return i >= s.lo && i <= s.hi; }
+
+__device__ __forceinline__ bool face_covers(const MpdBcProgram* bc, int f, int i) {
+    for (int s = 0; s < bc->nseg[f]; ++s)
+        if (seg_covers(bc->seg[f][s], i)) return true;
+    return false;
+}
+
+// The fixed-axis index of a face and its interior neighbour
+// (src/bcs/per_field_bc.cpp:264-274).
+__device__ __forceinline__ void face_nb(const StepArgs& a, int f, int l, int m, int& ln, int& mn) {
+    ln = l;
+    mn = m;
+    if (f == MPD_L_LO) ln = 2;
+    else if (f == MPD_L_HI) ln = a.L - 1;
+    else if (f == MPD_M_LO) mn = 2;
+    else mn = a.Mn - 1;
+}
+
+// Apply the segments [0, seg_end) of face f that cover cell (l, m) to the
+// physical state st, in order, with neighbour state nb: PerFieldBC::Apply's
+// per-cell physical part (src/bcs/per_field_bc.cpp:297-360, 391-446).
+// Returns the index of the last segment applied, or -1.
+__device__ int face_chain(const StepArgs& a, int f, int l, int m, int seg_end, double* st,
+                          const double* nb) {
+    const MpdBcProgram* bc = a.bc;
+    const bool is_l = (f == MPD_L_LO || f == MPD_L_HI);
+    const int i = is_l ? m : l;
+    int last = -1;
+    for (int s = 0; s < seg_end; ++s) {
+        const MpdSeg& g = bc->seg[f][s];
+        if (!seg_covers(g, i)) continue;
+        last = s;
+        double sym[MPD_BS_COUNT];
+        if (g.has_expr) {
+            int ln, mn;
+            face_nb(a, f, l, m, ln, mn);
+            sym[MPD_BS_GAMMA] = a.gamma;
+            sym[MPD_BS_BETA] = a.beta;
+            sym[MPD_BS_HZ0] = a.H_z0;
+            sym[MPD_BS_R0] = a.r_0;
+            sym[MPD_BS_DZ] = a.dz;
+            sym[MPD_BS_DR] = a.dr[l];
+            sym[MPD_BS_R] = cell_r(a, l, m);
+            sym[MPD_BS_RZ] = cell_rz(a, l, m);
+            sym[MPD_BS_Z] = (double)(l - 1) * a.dz;
+            sym[MPD_BS_RNB] = cell_r(a, ln, mn);
+            sym[MPD_BS_RZNB] = cell_rz(a, ln, mn);
+            for (int k = 0; k < 8; ++k) sym[MPD_BS_RHO_NB + k] = nb[kNbSlot[k]];
+        }
+        for (int k = 0; k < 8; ++k) {
+            const int slot = kCondSlot[k];
+            const MpdCond& c = g.c[k];
+            double v;
+            if (c.type == MPD_NEUMANN) v = nb[slot] + c.value;
+            else if (c.type == MPD_DIRICHLET) v = c.value;
+            else if (c.type == MPD_EXPRESSION) v = eval_expr(bc, c.code_off, c.code_len, sym);
+            else v = st[slot];
+            st[slot] = v;
+            if (g.has_expr) sym[MPD_BS_RHO + k] = v;
+        }
+    }
+    return last;
+}
+
+// RebuildUFromPhysical, include/physics_utils.hpp:15-24.
+__device__ __forceinline__ void rebuild_u(const double* ph, double r, double* u) {
+    u[0] = ph[PH_RHO] * r;
+    u[1] = ph[PH_RHO] * ph[PH_VZ] * r;
+    u[2] = ph[PH_RHO] * ph[PH_VR] * r;
+    u[3] = ph[PH_RHO] * ph[PH_VPHI] * r;
+    u[4] = ph[PH_RHO] * ph[PH_E] * r;
+    u[5] = ph[PH_HPHI];
+    u[6] = ph[PH_HZ] * r;
+    u[7] = ph[PH_HR] * r;
+}
+
+// State of boundary cell (l, m) as the M_LO face sees it while applying its
+// segment s (serial order, l ascending): the M_LO segments before s, plus
+// segment s itself when `with_s` (the cell was already visited in s).  Cell
+// (1,1) belongs to L_LO, which ran before; cell (L,1) to L_HI, which runs
+// after.
+__device__ void mlo_state(const StepArgs& a, int l, int s, bool with_s, double* ph) {
+    double nb[10];
+    if (l == 1) {
+        s0_phys(a, 1, 1, ph);
+        if (face_covers(a.bc, MPD_L_LO, 1)) {
+            s0_phys(a, 2, 1, nb);
+            face_chain(a, MPD_L_LO, 1, 1, a.bc->nseg[MPD_L_LO], ph, nb);
+        }
+        return;
+    }
+    s0_phys(a, l, 1, ph);
+    if (l == a.L) return;
+    s0_phys(a, l, 2, nb);
+    face_chain(a, MPD_M_LO, l, 1, with_s ? s + 1 : s, ph, nb);
+}
+
+// PerFieldBC::AxisLfU1/U2/U5/U7, src/bcs/per_field_bc.cpp:490-566, at (l, 1).
+// own: the cell's physical values after the segment; lo/hi: the states of
+// (l-1, 1) and (l+1, 1); c2: the state of (l, 2); sp: start-of-step physical
+// values at (l-1,1), (l,1), (l+1,1) (p and P are never written by the BCs).
+__device__ void axis_overrides(const StepArgs& a, const MpdSeg& g, int l, double dt, const double* own,
+                               const double* lo, const double* hi, const double* c2, const double* sp_lo,
+                               const double* sp_c, const double* sp_hi, double* u) {
+    const int m = 1;
+    const double r_n = cell_r(a, l + 1, m), r_s = cell_r(a, l - 1, m);
+    const double r_e = cell_r(a, l, m + 1), r_c = cell_r(a, l, m);
+    const double dr_l = a.dr[l], dz = a.dz;
+    auto U0 = [&](int k, int ll, int mm) { return fld(a.u_in, a.fstride, k)[at(a, ll, mm)]; };
+    if (g.c[0].type == MPD_AXIS_LF) {
+        const int k = 0;
+        const double n = U0(k, l + 1, m), s = U0(k, l - 1, m), e = U0(k, l, m + 1), c = U0(k, l, m);
+        u[0] = (0.25 * (n / r_n + s / r_s + e / r_e + c / r_c) +
+                dt * (-(n / r_n * hi[PH_VZ] - s / r_s * lo[PH_VZ]) / (2.0 * dz) -
+                      (e / r_e * c2[PH_VR] - c / r_c * c2[PH_VR]) / dr_l)) *
+               r_c;
+    }
+    if (g.c[1].type == MPD_AXIS_LF) {
+        const int k = 1;
+        const double n = U0(k, l + 1, m), s = U0(k, l - 1, m), e = U0(k, l, m + 1), c = U0(k, l, m);
+        u[1] = (0.25 * (n / r_n + s / r_s + e / r_e + c / r_c) +
+                dt * (((hi[PH_HZ] * hi[PH_HZ] - sp_hi[PH_PT]) - (lo[PH_HZ] * lo[PH_HZ] - sp_lo[PH_PT])) / (2.0 * dz) +
+                      (c2[PH_HZ] * c2[PH_HR] - own[PH_HZ] * own[PH_HR]) / dr_l -
+                      (n / r_n * hi[PH_VZ] - s / r_s * lo[PH_VZ]) / (2.0 * dz) -
+                      (e / r_e * c2[PH_VR] - c / r_c * own[PH_VR]) / dr_l)) *
+               r_c;
+    }
+    if (g.c[7].type == MPD_AXIS_LF) {
+        const int k = 4;
+        const double n = U0(k, l + 1, m), s = U0(k, l - 1, m), e = U0(k, l, m + 1), c = U0(k, l, m);
+        u[4] = (0.25 * (n / r_n + s / r_s + e / r_e + c / r_c) +
+                dt * (-sp_c[PH_P] * ((hi[PH_VZ] - lo[PH_VZ]) / (2.0 * dz) + (c2[PH_VR] - own[PH_VR]) / dr_l) -
+                      (n / r_n * hi[PH_VZ] - s / r_s * lo[PH_VZ]) / (2.0 * dz) -
+                      (e / r_e * c2[PH_VR] - c / r_c * own[PH_VR]) / dr_l)) *
+               r_c;
+    }
+    if (g.c[4].type == MPD_AXIS_LF) {
+        const int k = 6;
+        const double n = U0(k, l + 1, m), s = U0(k, l - 1, m), e = U0(k, l, m + 1), c = U0(k, l, m);
+        u[6] = (0.25 * (n / r_n + s / r_s + e / r_e + c / r_c) +
+                dt * ((c2[PH_HR] * c2[PH_VZ] - own[PH_HR] * own[PH_VZ]) / dr_l -
+                      (e / r_e * c2[PH_VR] - c / r_c * own[PH_VR]) / dr_l)) *
+               r_c;
+    }
+}
+
+// Final conserved values of boundary cell (l, m) written by face f (the last
+// face in L_LO, M_HI, M_LO, L_HI order that covers it).
+__device__ void bc_final(const StepArgs& a, int f, int l, int m, double dt, double* u) {
+    double st[10], nb[10];
+    // Own state before the face: only AxisLF conditions (M_LO only) read it.
+    s0_phys(a, l, m, st);
+    // Neighbour state before the face (src/face_bc.cpp order: L_LO, M_HI,
+    // M_LO, L_HI; only corners see an earlier face).
+    int ln, mn;
+    face_nb(a, f, l, m, ln, mn);
+    s0_phys(a, ln, mn, nb);
+    if (f == MPD_M_HI && ln == 1 && face_covers(a.bc, MPD_L_LO, mn)) {
+        double nb2[10];
+        s0_phys(a, 2, mn, nb2);
+        face_chain(a, MPD_L_LO, 1, mn, a.bc->nseg[MPD_L_LO], nb, nb2);
+    } else if (f == MPD_L_HI && mn == 1 && face_covers(a.bc, MPD_M_LO, ln)) {
+        double nb2[10];
+        s0_phys(a, ln, 2, nb2);
+        face_chain(a, MPD_M_LO, ln, 1, a.bc->nseg[MPD_M_LO], nb, nb2);
+    } else if (f == MPD_L_HI && mn == a.Mn && face_covers(a.bc, MPD_M_HI, ln)) {
+        double nb2[10];
+        s0_phys(a, ln, a.Mn - 1, nb2);
+        face_chain(a, MPD_M_HI, ln, a.Mn, a.bc->nseg[MPD_M_HI], nb, nb2);
+    }
+    const int last = face_chain(a, f, l, m, a.bc->nseg[f], st, nb);
+    const double r = cell_r(a, l, m);
+    rebuild_u(st, r, u);
+    if (f == MPD_M_LO && last >= 0 && a.bc->seg[f][last].has_axis) {
+        const MpdSeg& g = a.bc->seg[f][last];
+        double lo[10], hi[10], sp_lo[10], sp_c[10], sp_hi[10];
+        mlo_state(a, l - 1, last, seg_covers(g, l - 1), lo);
+        mlo_state(a, l + 1, last, false, hi);
+        start_phys(a, l - 1, 1, sp_lo);
+        start_phys(a, l, 1, sp_c);
+        start_phys(a, l + 1, 1, sp_hi);
+        axis_overrides(a, g, l, dt, st, lo, hi, nb, sp_lo, sp_c, sp_hi, u);
+    }
+}
+
+// dt bookkeeping of the time loop after one step: t += dt; dt =
+// ComputeDt(...) when adaptive (src/diagnostics.cpp:49-65); the next step's
+// dt truncated to T (src/main.cpp:220-223).
+__device__ void finalize_step(const StepArgs& a) {
+    MpdLoopState* st = a.st;
+    const uint64_t bits = __hip_atomic_load(&st->speed_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double speed = __longlong_as_double((long long)bits);
+    st->max_speed = speed;
+    const double dt_used = st->dt_use;
+    st->t = st->t + dt_used;
+    st->step = st->step + 1;
+    double dt = dt_used;
+    if (a.adaptive) {
+        const double dt_cfl = a.cfl_dx / (speed + 1.0e-10);
+        const double dt_grown = dt_used * a.growth;
+        const double v = (dt_grown < dt_cfl) ? dt_grown : dt_cfl;
+        dt = (v < a.dt_min) ? a.dt_min : ((a.dt_max < v) ? a.dt_max : v);
+    }
+    st->dt = dt;
+    st->dt_use = (st->t + dt > a.T) ? a.T - st->t : dt;
+    const bool cap = st->max_steps > 0 && st->step >= st->max_steps;
+    st->done = (!(st->t < a.T) || cap) ? 1 : 0;
+    st->speed_bits = 0;
+    st->ticket = 0;
+}
+
+// One thread per boundary cell: row 1, row L, column 1 (l = 2..L-1), column
+// Mn (l = 2..L-1).  Writes each cell's final conserved values, adds its wave
+// speed to the running max, and the last block finishes the step.
+__global__ void __launch_bounds__(256) lf_bc_ring(StepArgs a, double dt_explicit, int loop, int speed) {
+    if (loop && a.st->done) return;
+    const double dt = step_dt(a, dt_explicit, loop);
+    const int L = a.L, Mn = a.Mn;
+    const int nring = 2 * Mn + 2 * (L - 2);
+    const int idx = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    uint64_t key = 0;
+    if (idx < nring) {
+        int l, m;
+        if (idx < Mn) { l = 1; m = idx + 1; }
+        else if (idx < 2 * Mn) { l = L; m = idx - Mn + 1; }
+        else if (idx < 2 * Mn + L - 2) { l = idx - 2 * Mn + 2; m = 1; }
+        else { l = idx - (2 * Mn + L - 2) + 2; m = Mn; }
+        // Last face covering the cell (M_LO skips l = 1 and l = L by its corner
+        // policy, so it never owns a row-1 or row-L cell).
+        int f = -1;
+        const MpdBcProgram* bc = a.bc;
+        if (l == 1) {
+            if (m == Mn && face_covers(bc, MPD_M_HI, 1)) f = MPD_M_HI;
+            else if (face_covers(bc, MPD_L_LO, m)) f = MPD_L_LO;
+        } else if (l == L) {
+            if (face_covers(bc, MPD_L_HI, m)) f = MPD_L_HI;
+            else if (m == Mn && face_covers(bc, MPD_M_HI, L)) f = MPD_M_HI;
+        } else if (m == 1) {
+            if (face_covers(bc, MPD_M_LO, l)) f = MPD_M_LO;
+        } else {
+            if (face_covers(bc, MPD_M_HI, l)) f = MPD_M_HI;
+        }
+        double u[8];
+        bool have = true;
+        if (f >= 0) {
+            bc_final(a, f, l, m, dt, u);
+        } else if (m >= 2 && m <= Mn - 1) {
+            // Uncovered row cell keeps its central value.
+            const double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
+            for (int k = 0; k < 8; ++k) u[k] = rb[(int64_t)k * a.pitch + m];
+        } else {
+            // Uncovered column cell: the reference never writes u there, so it
+            // stays 0 (zero-initialised); the buffers already hold 0.
+            have = false;
+            for (int k = 0; k < 8; ++k) u[k] = 0.0;
+        }
+        if (have) {
+            const int64_t i = at(a, l, m);
+            for (int k = 0; k < 8; ++k) a.u_out[(int64_t)k * a.fstride + i] = u[k];
+        }
+        if (speed) key = speed_key(wave_speed_u(u, cell_r(a, l, m), a.gamma, a.gm1));
+    }
+    if (speed) {
+        key = wave_max_u64(key);
+        if ((threadIdx.x & 63) == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
+    }
+    if (!loop) return;
+    // Last block finalises the loop state.
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(&a.st->ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        finalize_step(a);
+    }
+}
+
+// -------------------------------------------------------------- reductions
+
+// SolutionChange over the 7 fields rho, v_z, v_r, v_phi, H_z, H_r, H_phi of the
+// interior, then SavePrev.  prev: 7 fields, same layout.  partial: 2 doubles
+// per block.  The last block sums the partials in block order.
+__global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, double* partial,
+                                                   int expected_step, double threshold) {
+    if (a.st->step != expected_step) return;
+    const int L = a.L, Mn = a.Mn;
+    const int64_t n = (int64_t)L * Mn;
+    double sd = 0.0, sc = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int l = 1 + (int)(c / Mn), m = 1 + (int)(c % Mn);
+        const int64_t i = at(a, l, m);
+        double u[8], ph[10];
+        for (int k = 0; k < 8; ++k) u[k] = a.u_in[(int64_t)k * a.fstride + i];
+        phys10_from_u(u, cell_r(a, l, m), a.gm1, ph);
+        const int ids[7] = {PH_RHO, PH_VZ, PH_VR, PH_VPHI, PH_HZ, PH_HR, PH_HPHI};
+        double dsum = 0.0, csum = 0.0;
+        for (int k = 0; k < 7; ++k) {
+            const double cur = ph[ids[k]];
+            const double d = cur - prev[(int64_t)k * a.fstride + i];
+            dsum += d * d;
+            csum += cur * cur;
+            prev[(int64_t)k * a.fstride + i] = cur;
+        }
+        sd += dsum;
+        sc += csum;
+    }
+    __shared__ double sh_d[256], sh_c[256];
+    sh_d[threadIdx.x] = sd;
+    sh_c[threadIdx.x] = sc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh_d[threadIdx.x] += sh_d[threadIdx.x + s];
+            sh_c[threadIdx.x] += sh_c[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sh_d[0];
+        partial[2 * blockIdx.x + 1] = sh_c[0];
+        __threadfence();
+        const unsigned t = atomicAdd(&a.st->ticket2, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double gd = 0.0, gc = 0.0;
+        for (unsigned b = 0; b < gridDim.x; ++b) {
+            gd += __hip_atomic_load(&partial[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            gc += __hip_atomic_load(&partial[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const double nd = sqrt(gd), nc = sqrt(gc);
+        const double change = (nc > 1e-15) ? (nd / nc) : nd;
+        a.st->change = change;
+        if (threshold > 0.0 && change < threshold) {
+            a.st->converged = 1;
+            a.st->done = 1;
+        }
+        a.st->ticket2 = 0;
+    }
+}
+
+// Max wave speed of the state u_in over the interior (or of the IC physical
+// arrays on the first step).
+__global__ void __launch_bounds__(256) lf_max_speed(StepArgs a, unsigned long long* out) {
+    const int64_t n = (int64_t)a.L * a.Mn;
+    uint64_t key = 0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int l = 1 + (int)(c / a.Mn), m = 1 + (int)(c % a.Mn);
+        double ph[10];
+        start_phys(a, l, m, ph);
+        const uint64_t k = speed_key(wave_speed_ph(ph, a.gamma));
+        key = (k > key) ? k : key;
+    }
+    key = wave_max_u64(key);
+    if ((threadIdx.x & 63) == 0 && key != 0) atomicMax(out, (unsigned long long)key);
+}
+
+// Physical fields of the state (10 fields, ghosts 0): the arrays the reference
+// keeps in Fields after a step.
+__global__ void __launch_bounds__(256) lf_phys_fields(StepArgs a, double* out) {
+    const int64_t n = (int64_t)(a.L + 2) * a.pitch;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int l = (int)(i / a.pitch), m = (int)(i % a.pitch);
+        double ph[10];
+        if (l >= 1 && l <= a.L && m >= 1 && m <= a.Mn) {
+            start_phys(a, l, m, ph);
+        } else {
+            for (int k = 0; k < 10; ++k) ph[k] = 0.0;
+        }
+        for (int k = 0; k < 10; ++k) out[(int64_t)k * a.fstride + i] = ph[k];
+    }
+}
+
+// Verification::ComputeIntegrals (src/verification.cpp:16-45): per-row sums
+// in m order, rows combined in l order by the last block.  rowsum: 7*(L+2).
+__global__ void __launch_bounds__(64) lf_integrals(StepArgs a, double* rowsum, double* out) {
+    const int l = 1 + (int)blockIdx.x;
+    __shared__ double sh[7][64];
+    const double area = a.dr[l] * a.dz;
+    const int ks[7] = {0, 1, 2, 3, 4, 6, 5};
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int m = 1 + (int)threadIdx.x; m <= a.Mn; m += 64) {
+        const int64_t i = at(a, l, m);
+        for (int k = 0; k < 7; ++k) acc[k] += a.u_in[(int64_t)ks[k] * a.fstride + i] * area;
+    }
+    for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        double s = 0.0;
+        for (int t = 0; t < 64; ++t) s += sh[threadIdx.x][t];
+        rowsum[(int64_t)threadIdx.x * (a.L + 2) + l] = s;
+    }
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(&a.st->ticket2, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x < 7) {
+        __threadfence();
+        double s = 0.0;
+        for (int ll = 1; ll <= a.L; ++ll)
+            s += __hip_atomic_load(&rowsum[(int64_t)threadIdx.x * (a.L + 2) + ll], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out[threadIdx.x] = s;
+    }
+    if (last && threadIdx.x == 0) a.st->ticket2 = 0;
+}
+
+// GetMassFlux / GetThrust (src/integrals.cpp:44-112) in the reference's serial
+// m order: one thread.  out[0] = mass flux, out[1] = thrust.
+__global__ void lf_outlet(StepArgs a, int M_max, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int l = a.L;
+    const double dr_out = a.dr[l];
+    const double pi = 3.14159265358979323846;
+    const double inv_8pi = 1.0 / (8.0 * pi);
+    double mf = 0.0, th = 0.0;
+    for (int m = 1; m <= a.Mn; ++m) {
+        const int mg = m - 1;
+        const double w = (mg == 0 || mg == M_max) ? 0.5 : 1.0;
+        double ph[10];
+        start_phys(a, l, m, ph);
+        const double r = cell_r(a, l, m);
+        const double f1 = ph[PH_RHO] * ph[PH_VZ];
+        const double v_z2 = ph[PH_VZ] * ph[PH_VZ];
+        const double H2 = ph[PH_HZ] * ph[PH_HZ] + ph[PH_HR] * ph[PH_HR] + ph[PH_HPHI] * ph[PH_HPHI];
+        const double f2 = ph[PH_RHO] * v_z2 + ph[PH_P] + H2 * inv_8pi;
+        mf += w * (f1 * 2.0 * pi * r) * dr_out;
+        th += w * (f2 * 2.0 * pi * r) * dr_out;
+    }
+    out[0] = mf;
+    out[1] = th;
+}
+
+// Verification::CheckRadialSymmetry (src/verification.cpp:120-146).
+__global__ void __launch_bounds__(256) lf_radial_sym(StepArgs a, unsigned long long* out) {
+    const int64_t n = (int64_t)a.L * (a.Mn - 1);
+    uint64_t key = 0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int l = 1 + (int)(c / (a.Mn - 1)), m = 1 + (int)(c % (a.Mn - 1));
+        double pa[10], pb[10];
+        start_phys(a, l, m, pa);
+        start_phys(a, l, m + 1, pb);
+        const double x = pa[PH_RHO], y = pb[PH_RHO];
+        const double mean = 0.5 * (x + y);
+        if (mean > 1.0e-15) {
+            const uint64_t k = speed_key(fabs(y - x) / mean);
+            key = (k > key) ? k : key;
+        }
+    }
+    key = wave_max_u64(key);
+    if ((threadIdx.x & 63) == 0 && key != 0) atomicMax(out, (unsigned long long)key);
+}
+
+// Fill kernel (used instead of memset nodes inside captured graphs).
+__global__ void lf_fill(double* p, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template __global__ void lf_stencil_simple<true, true, DivExact>(StepArgs, double, int);
+template __global__ void lf_stencil_simple<true, false, DivExact>(StepArgs, double, int);
+template __global__ void lf_stencil_simple<false, true, DivExact>(StepArgs, double, int);
+template __global__ void lf_stencil_simple<false, false, DivExact>(StepArgs, double, int);
+template __global__ void lf_stencil_simple<false, true, DivConst>(StepArgs, double, int);
+template __global__ void lf_stencil_simple<false, false, DivConst>(StepArgs, double, int);
+
+}  // namespace mpd
