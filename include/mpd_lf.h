@@ -1,0 +1,122 @@
+/*
+ * mpd_lf.h -- C ABI of the MI355X (gfx950) explicit Lax-Friedrichs solver.
+ *
+ * The library (mpd-plasma-dynamics_amd/libmpd_lf.so) is a drop-in for the
+ * reference's time step, Solver::Advance (include/solver.hpp:24-53,
+ * src/solver.cpp:22-37), and for the reductions the reference's driver runs
+ * on the same Fields between steps.  The whole (z, r) domain lives on one GPU.
+ *
+ * Conventions: every call returns 0 on success and a negative MPD_E* code on
+ * failure (mpd_last_error() gives the message of the calling thread's last
+ * failure); no C++ exception crosses this boundary.  Host arrays are the
+ * reference Fields layout: (L_max + 2) rows x (M_max + 3) columns of doubles,
+ * row-major, ghost ring included (include/fields.hpp:10-22).  Calls on one
+ * handle are stream-ordered and must come from one thread at a time; calls
+ * that return numbers synchronise.
+ */
+#ifndef MPD_LF_H
+#define MPD_LF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPD_OK 0
+#define MPD_E_CONFIG -1   /* bad configuration (the reference throws std::runtime_error) */
+#define MPD_E_DEVICE -2   /* HIP runtime / allocation failure (the reference aborts) */
+#define MPD_E_ARG -3      /* bad argument */
+#define MPD_E_STATE -4    /* non-finite or invalid state detected */
+
+typedef struct mpd_solver mpd_solver;
+
+/* Result of mpd_run: the loop variables of src/main.cpp:211-247. */
+typedef struct {
+    double t;        /* simulated time after the run */
+    double dt;       /* dt variable of the loop (next step's dt before truncation) */
+    double change;   /* last SolutionChange value, -1 if none was computed */
+    int steps;       /* steps taken by this call */
+    int total_steps; /* steps taken since creation */
+    int converged;   /* SolutionChange fell below convergence.threshold */
+    int done;        /* the loop condition is false (t >= T, converged, or step cap) */
+} mpd_run_result;
+
+/* Build a solver from a config.yaml path / text: SimConfig::Load
+ * (src/config.cpp:261-384), GeometryRegistry::Create, ExpressionIC, Grid,
+ * Fields::InitPhysical/InitConservative and the Solver constructor
+ * (src/main.cpp:28-185; src/solver.cpp:9-16).  The IC state is uploaded. */
+int mpd_create_from_file(const char* yaml_path, mpd_solver** out);
+int mpd_create_from_string(const char* yaml_text, mpd_solver** out);
+void mpd_destroy(mpd_solver* s);
+const char* mpd_last_error(void);
+
+/* Grid sizes: L_max, M_max, rows = L_max + 2, cols = M_max + 3. */
+int mpd_dims(const mpd_solver* s, int* L_max, int* M_max, int* rows, int* cols);
+
+/* Replace the state.  u0: 8 arrays (u0_1..u0_8, Fields::u0_*); phys: 10
+ * arrays in the Fields member order rho, v_z, v_r, v_phi, e, p, P, H_z, H_r,
+ * H_phi, or NULL to take physical = f(u0).  When given, the next step reads
+ * these physical values directly, as the reference's first step reads the IC
+ * (src/main.cpp:147-148).  Ghost cells are taken as zero. */
+int mpd_upload_state(mpd_solver* s, const double* const* u0, const double* const* phys);
+
+/* Copy the state out: u0 (8 arrays) and/or the physical arrays (10), either
+ * may be NULL.  Physical values are those Fields holds after the last step. */
+int mpd_download(mpd_solver* s, double* const* u0, double* const* phys);
+
+/* Solver::Advance(dt), src/solver.cpp:22-37: one step with the given dt. */
+int mpd_advance(mpd_solver* s, double dt);
+
+/* Diagnostics::ComputeDt (src/diagnostics.cpp:49-65) and MaxWaveSpeed
+ * (src/diagnostics.cpp:14-43) on the current state. */
+int mpd_compute_dt(mpd_solver* s, double dt_current, double* dt_out);
+int mpd_max_wave_speed(mpd_solver* s, double* out);
+
+/* Diagnostics::SolutionChange (src/diagnostics.cpp:71-106) against the
+ * snapshot, then Fields::SavePrev (src/fields.cpp:59-74) when save_prev. */
+int mpd_solution_change(mpd_solver* s, int save_prev, double* out);
+
+/* The time loop of src/main.cpp:211-247 (dt policy, T truncation,
+ * convergence check every check_frequency steps), continued from the
+ * solver's loop state, for at most max_steps steps (<= 0: no cap besides
+ * time.max_steps).  Runs as device-resident HIP graphs. */
+int mpd_run(mpd_solver* s, int max_steps, mpd_run_result* out);
+
+/* Verification::ComputeIntegrals (src/verification.cpp:16-45): mass, mom_z,
+ * mom_r, mom_phi, energy, mag_z, mag_phi. */
+int mpd_integrals(mpd_solver* s, double out7[7]);
+
+/* Diagnostics::GetMassFlux / GetThrust (src/integrals.cpp:44-112). */
+int mpd_outlet(mpd_solver* s, double* mass_flux, double* thrust);
+
+/* Verification::CheckRadialSymmetry (src/verification.cpp:120-146). */
+int mpd_radial_symmetry(mpd_solver* s, double* out);
+
+/* Grid arrays (Grid::Build, src/grid.cpp:20-42): r and r_z as rows x cols,
+ * R and dr as rows entries, r_0.  Any pointer may be NULL. */
+int mpd_grid(const mpd_solver* s, double* r, double* r_z, double* R, double* dr, double* r_0);
+
+/* Host-only (no GPU needed): parse and validate a config (SimConfig::Load,
+ * geometry, IC and BC compilation) and report the grid sizes. */
+int mpd_config_check(const char* yaml_text, int* L_max, int* M_max);
+
+/* Host-only: the step-0 state the solver starts from (ExpressionIC::Apply,
+ * src/ics/expression_ic.cpp:286-324, then Fields::InitConservative,
+ * src/fields.cpp:40-55) into caller arrays of rows x cols (either may be
+ * NULL). */
+int mpd_initial_state(const char* yaml_text, double* const* u0, double* const* phys);
+
+/* Kernel selection: key "kernel" (0 = simple, 1 = tiled), "fast_div" (0/1),
+ * "tile_rows", "steps_per_graph". */
+int mpd_set_option(mpd_solver* s, const char* key, double value);
+
+/* Benchmark helper: run `warmup` then `steps` loop steps with the current
+ * options and return HIP-event times measured on the solver's stream:
+ * total_ms for the timed steps, and per-launch averages of the stencil and
+ * boundary kernels from a separately event-bracketed pass of `steps` steps. */
+int mpd_time_steps(mpd_solver* s, int steps, int warmup, double* total_ms, double* stencil_ms,
+                   double* bc_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

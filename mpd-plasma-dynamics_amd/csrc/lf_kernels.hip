@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "lf_device.h"
+#include "lf_launch.h"
 
 namespace mpd {
 
@@ -125,6 +126,110 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
     if (kSpeed) {
         key = wave_max_u64(key);
         if ((threadIdx.x & 63) == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
+    }
+}
+
+// Record of row l at column m for the marching kernel; zero physical values on
+// the ghost rows and beyond the last interior column.
+__device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, bool mload, const double* raw, Rec& c) {
+    for (int k = 0; k < 8; ++k) c.u[k] = raw[k];
+    c.r = cell_r(a, l, m);
+    if (l == 0 || l == a.L + 1 || m > a.Mn) {
+        c.rho = c.vz = c.vr = c.vphi = c.p = c.P = c.Hz = c.Hr = c.Hphi = 0.0;
+    } else {
+        rec_phys(c, a.gm1);
+    }
+    (void)mload;
+}
+
+__device__ __forceinline__ void load_raw(const StepArgs& a, int l, int m, bool mload, double* raw) {
+    if (mload) {
+        const int64_t i = at(a, l, m);
+        for (int k = 0; k < 8; ++k) raw[k] = fld(a.u_in, a.fstride, k)[i];
+    } else {
+        for (int k = 0; k < 8; ++k) raw[k] = 0.0;
+    }
+}
+
+__device__ __forceinline__ void shfl_rec(const Rec& c, Rec& e, Rec& w) {
+    for (int k = 0; k < 8; ++k) {
+        e.u[k] = __shfl_down(c.u[k], 1, 64);
+        w.u[k] = __shfl_up(c.u[k], 1, 64);
+    }
+    e.vz = __shfl_down(c.vz, 1, 64);     w.vz = __shfl_up(c.vz, 1, 64);
+    e.vr = __shfl_down(c.vr, 1, 64);     w.vr = __shfl_up(c.vr, 1, 64);
+    e.vphi = __shfl_down(c.vphi, 1, 64); w.vphi = __shfl_up(c.vphi, 1, 64);
+    e.P = __shfl_down(c.P, 1, 64);       w.P = __shfl_up(c.P, 1, 64);
+    e.Hz = __shfl_down(c.Hz, 1, 64);     w.Hz = __shfl_up(c.Hz, 1, 64);
+    e.Hr = __shfl_down(c.Hr, 1, 64);     w.Hr = __shfl_up(c.Hr, 1, 64);
+    e.Hphi = __shfl_down(c.Hphi, 1, 64); w.Hphi = __shfl_up(c.Hphi, 1, 64);
+    e.r = __shfl_down(c.r, 1, 64);       w.r = __shfl_up(c.r, 1, 64);
+}
+
+// The production stencil.  Each wavefront owns a tile of 64 columns (62
+// outputs plus one halo column on each side) by H rows and marches down the
+// rows keeping the records of rows l-1, l, l+1 in registers: every cell's
+// conserved values are read from HBM once and its physical values computed
+// once; the m +- 1 neighbours come from the adjacent lanes (ds_bpermute),
+// the l +- 1 neighbours from the register window; the next row's loads are
+// issued before the current row's arithmetic.  Tiles are ordered
+// column-strip-fastest so neighbouring waves share halo columns in L2.
+template <bool kSpeed, class Div>
+__global__ void __launch_bounds__(256) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
+                                                        int nstrips, int ntiles) {
+    if (loop && a.st->done) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int tile = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (tile >= ntiles) return;
+    const int strip = tile % nstrips, chunk = tile / nstrips;
+    const int m = 1 + 62 * strip + lane;
+    const int l0 = 1 + chunk * H;
+    const int l1 = min(l0 + H, a.L + 1);
+    const double dt = step_dt(a, dt_explicit, loop);
+    const bool mload = m <= a.Mn + 1;
+    const bool outlane = lane >= 1 && lane <= 62 && m <= a.Mn - 1;
+    const double dz2 = a.dz2, dz2i = 1.0 / a.dz2;
+    Rec S, C, N;
+    {
+        double raw[8];
+        load_raw(a, l0 - 1, m, mload, raw);
+        row_rec(a, l0 - 1, m, mload, raw, S);
+        load_raw(a, l0, m, mload, raw);
+        row_rec(a, l0, m, mload, raw, C);
+        load_raw(a, l0 + 1, m, mload, raw);
+        row_rec(a, l0 + 1, m, mload, raw, N);
+    }
+    uint64_t key = 0;
+    for (int l = l0; l < l1; ++l) {
+        const bool pre = (l + 2 <= l1);
+        double raw[8];
+        if (pre) load_raw(a, l + 2, m, mload, raw);
+        Rec E, W;
+        shfl_rec(C, E, W);
+        const double dr2 = 2 * a.dr[l];
+        const double dr2i = 1.0 / dr2;
+        double o[8];
+        lf_cell<Div>(N, S, E, W, C, dt, dz2, dz2i, dr2, dr2i, o);
+        if (outlane) {
+            if (l == 1 || l == a.L) {
+                double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
+                for (int k = 0; k < 8; ++k) rb[(int64_t)k * a.pitch + m] = o[k];
+            } else {
+                const int64_t i = at(a, l, m);
+                for (int k = 0; k < 8; ++k) __builtin_nontemporal_store(o[k], a.u_out + (int64_t)k * a.fstride + i);
+                if (kSpeed) {
+                    const uint64_t kk = speed_key(wave_speed_u(o, C.r, a.gamma, a.gm1));
+                    key = (kk > key) ? kk : key;
+                }
+            }
+        }
+        S = C;
+        C = N;
+        if (pre) row_rec(a, l + 2, m, mload, raw, N);
+    }
+    if (kSpeed) {
+        key = wave_max_u64(key);
+        if (lane == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
     }
 }
 
@@ -486,20 +591,25 @@ __global__ void __launch_bounds__(256) lf_bc_ring(StepArgs a, double dt_explicit
 // -------------------------------------------------------------- reductions
 
 // SolutionChange over the 7 fields rho, v_z, v_r, v_phi, H_z, H_r, H_phi of the
-// interior, then SavePrev.  prev: 7 fields, same layout.  partial: 2 doubles
-// per block.  The last block sums the partials in block order.
-__global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, double* partial,
-                                                   int expected_step, double threshold) {
-    if (a.st->step != expected_step) return;
+// interior, then SavePrev when `save`.  prev: 7 fields, same layout.
+// partial: 2 doubles per block, summed in block order by the last block.
+// In the loop (`loop`), it runs only on a step that is a multiple of
+// check_frequency and was not checked yet, and sets converged/done
+// (src/main.cpp:233-247).
+__global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, double* partial, int loop,
+                                                   int check_frequency, double threshold, int save) {
+    if (loop) {
+        const int step = a.st->step;
+        if (step % check_frequency != 0 || step == a.st->checked_step) return;
+    }
     const int L = a.L, Mn = a.Mn;
     const int64_t n = (int64_t)L * Mn;
     double sd = 0.0, sc = 0.0;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         const int l = 1 + (int)(c / Mn), m = 1 + (int)(c % Mn);
         const int64_t i = at(a, l, m);
-        double u[8], ph[10];
-        for (int k = 0; k < 8; ++k) u[k] = a.u_in[(int64_t)k * a.fstride + i];
-        phys10_from_u(u, cell_r(a, l, m), a.gm1, ph);
+        double ph[10];
+        start_phys(a, l, m, ph);
         const int ids[7] = {PH_RHO, PH_VZ, PH_VR, PH_VPHI, PH_HZ, PH_HR, PH_HPHI};
         double dsum = 0.0, csum = 0.0;
         for (int k = 0; k < 7; ++k) {
@@ -507,7 +617,7 @@ __global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, dou
             const double d = cur - prev[(int64_t)k * a.fstride + i];
             dsum += d * d;
             csum += cur * cur;
-            prev[(int64_t)k * a.fstride + i] = cur;
+            if (save) prev[(int64_t)k * a.fstride + i] = cur;
         }
         sd += dsum;
         sc += csum;
@@ -542,9 +652,12 @@ __global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, dou
         const double nd = sqrt(gd), nc = sqrt(gc);
         const double change = (nc > 1e-15) ? (nd / nc) : nd;
         a.st->change = change;
-        if (threshold > 0.0 && change < threshold) {
-            a.st->converged = 1;
-            a.st->done = 1;
+        if (loop) {
+            a.st->checked_step = a.st->step;
+            if (change < threshold) {
+                a.st->converged = 1;
+                a.st->done = 1;
+            }
         }
         a.st->ticket2 = 0;
     }
@@ -670,11 +783,104 @@ __global__ void lf_fill(double* p, int64_t n, double v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-template __global__ void lf_stencil_simple<true, true, DivExact>(StepArgs, double, int);
-template __global__ void lf_stencil_simple<true, false, DivExact>(StepArgs, double, int);
-template __global__ void lf_stencil_simple<false, true, DivExact>(StepArgs, double, int);
-template __global__ void lf_stencil_simple<false, false, DivExact>(StepArgs, double, int);
-template __global__ void lf_stencil_simple<false, true, DivConst>(StepArgs, double, int);
-template __global__ void lf_stencil_simple<false, false, DivConst>(StepArgs, double, int);
+// ---------------------------------------------------------------- launchers
+
+namespace {
+template <class Div>
+hipError_t launch_simple(const StepArgs& a, double dt, int loop, int speed, hipStream_t s) {
+    const dim3 blk(64, 4);
+    const dim3 grd((unsigned)((a.Mn - 2 + 63) / 64), (unsigned)((a.L + 3) / 4));
+    if (a.step1) {
+        if (speed) hipLaunchKernelGGL((lf_stencil_simple<true, true, Div>), grd, blk, 0, s, a, dt, loop);
+        else hipLaunchKernelGGL((lf_stencil_simple<true, false, Div>), grd, blk, 0, s, a, dt, loop);
+    } else {
+        if (speed) hipLaunchKernelGGL((lf_stencil_simple<false, true, Div>), grd, blk, 0, s, a, dt, loop);
+        else hipLaunchKernelGGL((lf_stencil_simple<false, false, Div>), grd, blk, 0, s, a, dt, loop);
+    }
+    return hipGetLastError();
+}
+
+template <class Div>
+hipError_t launch_tiled(const StepArgs& a, int H, double dt, int loop, int speed, hipStream_t s) {
+    const int nstrips = (a.Mn - 2 + 61) / 62;
+    const int nchunks = (a.L + H - 1) / H;
+    const int ntiles = nstrips * nchunks;
+    const unsigned blocks = (unsigned)((ntiles + 3) / 4);
+    if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+    else hipLaunchKernelGGL((lf_stencil_tiled<false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+    return hipGetLastError();
+}
+
+int auto_rows(const StepArgs& a) {
+    // Enough tiles to give every CU several waves, rows long enough that the
+    // two halo rows per tile stay a small fraction of the work.
+    const int nstrips = (a.Mn - 2 + 61) / 62;
+    const long long target_tiles = 256LL * 4 * 6;
+    long long H = ((long long)a.L * nstrips + target_tiles - 1) / target_tiles;
+    if (H < 16) H = 16;
+    if (H > 512) H = 512;
+    return (int)H;
+}
+}  // namespace
+
+hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt, int loop, int speed, hipStream_t s) {
+    if (a.step1 || c.kind == STENCIL_SIMPLE) {
+        // The first step reads the IC physical arrays: always the simple kernel.
+        if (c.fast_div && !a.step1) return launch_simple<DivConst>(a, dt, loop, speed, s);
+        return launch_simple<DivExact>(a, dt, loop, speed, s);
+    }
+    const int H = c.tile_rows > 0 ? c.tile_rows : auto_rows(a);
+    if (c.fast_div) return launch_tiled<DivConst>(a, H, dt, loop, speed, s);
+    return launch_tiled<DivExact>(a, H, dt, loop, speed, s);
+}
+
+hipError_t launch_bc(const StepArgs& a, double dt, int loop, int speed, hipStream_t s) {
+    const int nring = 2 * a.Mn + 2 * (a.L - 2);
+    hipLaunchKernelGGL(lf_bc_ring, dim3((unsigned)((nring + 255) / 256)), dim3(256), 0, s, a, dt, loop, speed);
+    return hipGetLastError();
+}
+
+int residual_blocks(const StepArgs& a) {
+    const long long n = (long long)a.L * a.Mn;
+    long long b = (n + 255) / 256;
+    return (int)(b < 1024 ? b : 1024);
+}
+
+hipError_t launch_residual(const StepArgs& a, double* prev, double* partial, int loop, int check_frequency,
+                           double threshold, int save, hipStream_t s) {
+    hipLaunchKernelGGL(lf_residual, dim3((unsigned)residual_blocks(a)), dim3(256), 0, s, a, prev, partial, loop,
+                       check_frequency, threshold, save);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_speed(const StepArgs& a, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(lf_max_speed, dim3(1024), dim3(256), 0, s, a, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_phys_fields(const StepArgs& a, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(lf_phys_fields, dim3(1024), dim3(256), 0, s, a, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrals(const StepArgs& a, double* rowsum, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(lf_integrals, dim3((unsigned)a.L), dim3(64), 0, s, a, rowsum, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_outlet(const StepArgs& a, int M_max, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(lf_outlet, dim3(1), dim3(64), 0, s, a, M_max, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_radial_sym(const StepArgs& a, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(lf_radial_sym, dim3(1024), dim3(256), 0, s, a, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, long long n, double v, hipStream_t s) {
+    hipLaunchKernelGGL(lf_fill, dim3(1024), dim3(256), 0, s, p, (int64_t)n, v);
+    return hipGetLastError();
+}
 
 }  // namespace mpd

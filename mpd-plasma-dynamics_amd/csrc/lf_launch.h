@@ -1,0 +1,35 @@
+// lf_launch.h -- host-side launchers of the kernels in lf_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lf_device.h"
+
+namespace mpd {
+
+enum StencilKind { STENCIL_SIMPLE = 0, STENCIL_TILED = 1 };
+
+struct StencilCfg {
+    int kind = STENCIL_TILED;
+    bool fast_div = true;
+    int tile_rows = 0;  // rows per wave of the tiled kernel (0: chosen from the grid)
+};
+
+// Central update (+ wave-speed epilogue when speed).  loop: read dt and the
+// done flag from the device loop state instead of dt_explicit.
+hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt_explicit, int loop, int speed,
+                          hipStream_t s);
+// Boundary faces; with loop the last block finishes the step's loop state.
+hipError_t launch_bc(const StepArgs& a, double dt_explicit, int loop, int speed, hipStream_t s);
+// SolutionChange (+ SavePrev when save).  loop: run only on a check step.
+hipError_t launch_residual(const StepArgs& a, double* prev, double* partial, int loop, int check_frequency,
+                           double threshold, int save, hipStream_t s);
+int residual_blocks(const StepArgs& a);
+hipError_t launch_max_speed(const StepArgs& a, unsigned long long* out, hipStream_t s);
+hipError_t launch_phys_fields(const StepArgs& a, double* out, hipStream_t s);
+hipError_t launch_integrals(const StepArgs& a, double* rowsum, double* out, hipStream_t s);
+hipError_t launch_outlet(const StepArgs& a, int M_max, double* out, hipStream_t s);
+hipError_t launch_radial_sym(const StepArgs& a, unsigned long long* out, hipStream_t s);
+hipError_t launch_fill(double* p, long long n, double v, hipStream_t s);
+
+}  // namespace mpd

@@ -81,4 +81,6 @@ struct MpdLoopState {
     uint64_t speed_bits;  // running max of the wave speed (bit pattern, values >= 0)
     uint32_t ticket;      // completion counter for last-block finalisation
     uint32_t ticket2;
+    int32_t checked_step;  // step at which SolutionChange last ran in the loop
+    int32_t pad;
 };

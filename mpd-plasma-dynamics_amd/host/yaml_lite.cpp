@@ -2,6 +2,8 @@
 #include "yaml_lite.h"
 
 #include <cctype>
+#include <cmath>
+#include <cstring>
 #include <cerrno>
 #include <cstdlib>
 #include <fstream>

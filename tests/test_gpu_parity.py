@@ -1,0 +1,204 @@
+"""GPU parity: the HIP library against the CPU oracle on the same configs.
+
+Bar: bit-identical results (NaN matches NaN) for every conserved and physical
+field, t, dt and the step count.  Where the reference itself is not
+bit-determined (SolutionChange and the conserved integrals sum in OpenMP /
+tree order) the tolerance is written in the test.  All arithmetic is FP64;
+the stated parity tolerance of the LF step is 1e-12 relative L-infinity per
+field, which bit-identity meets with room to spare.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import cases
+from parity import describe_mismatch, rel_linf, run_oracle, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare_state(sol, o, label):
+    u, p = sol.download()
+    for k in range(8):
+        assert same_bits(u[k], o.u0(k)), f"{label} u0_{k + 1}: {describe_mismatch(u[k], o.u0(k))}"
+    for k in range(10):
+        assert same_bits(p[k], o.phys(k)), f"{label} {O.PHYS_NAMES[k]}: {describe_mismatch(p[k], o.phys(k))}"
+
+
+CASES = {
+    "straight_40x16": lambda **kw: cases.channel_config(40, 16, max_steps=60, **kw),
+    "condiv_noise_70x130": lambda **kw: cases.channel_config(70, 130, geometry="condiv", noise=1e-3,
+                                                             max_steps=40, **kw),
+    "thruster_80x40": lambda **kw: cases.thruster_config(80, 40, max_steps=60, **kw),
+    "mixed_24x14": lambda **kw: cases.mixed_config(24, 14, max_steps=30, **kw),
+    "minimal_3x2": lambda **kw: cases.channel_config(3, 2, max_steps=12, **kw),
+}
+
+
+@pytest.mark.parametrize("kernel,fast_div", [("simple", False), ("simple", True), ("tiled", False),
+                                             ("tiled", True)])
+@pytest.mark.parametrize("name", list(CASES))
+def test_run_bit_exact(mpd, name, kernel, fast_div):
+    text = CASES[name](kernel=kernel, fast_div=fast_div)
+    print(f"case {name} kernel={kernel} fast_div={fast_div}", flush=True)
+    o, ores = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        res = sol.run()
+        assert res["steps"] == ores["steps"]
+        assert res["t"] == ores["t"], (res["t"], ores["t"])
+        assert res["dt"] == ores["dt"], (res["dt"], ores["dt"])
+        _compare_state(sol, o, name)
+
+
+def test_tiled_odd_tile_rows(mpd):
+    """Tile heights that do not divide L, one-row tiles and a strip count
+    that leaves a mostly empty last strip."""
+    for h in (1, 5, 7, 64):
+        text = cases.channel_config(37, 125, geometry="condiv", noise=1e-3, max_steps=9, tile_rows=h)
+        print(f"tile_rows={h}", flush=True)
+        o, _ = run_oracle(text, 0)
+        with mpd.Solver(text) as sol:
+            sol.run()
+            _compare_state(sol, o, f"tile_rows={h}")
+
+
+def test_advance_and_compute_dt_match_oracle(mpd):
+    """Solver::Advance(dt) + Diagnostics::ComputeDt driven from the host, as
+    src/main.cpp:225-231 drives them."""
+    text = cases.thruster_config(60, 30, max_steps=0)
+    o = O.Oracle(cases.oracle_case(text))
+    with mpd.Solver(text) as sol:
+        dt = dt_o = 1.0e-7
+        for step in range(25):
+            sol.advance(dt)
+            o.advance(dt_o)
+            dt = sol.compute_dt(dt)
+            dt_o = o.compute_dt(dt_o)
+            assert dt == dt_o, (step, dt, dt_o)
+        assert sol.max_wave_speed() == o.max_wave_speed()
+        _compare_state(sol, o, "advance")
+
+
+def test_run_resumes_across_calls(mpd):
+    """Several run() calls (graph relaunches with eager work in between) give
+    the same state as one call."""
+    text = cases.channel_config(50, 20, max_steps=0, geometry="condiv")
+    o = O.Oracle(cases.oracle_case(text))
+    ores = o.run(77)
+    with mpd.Solver(text) as sol:
+        for n in (1, 10, 3, 40, 23):
+            sol.run(n)
+            sol.integrals()
+            sol.outlet()
+        u, _ = sol.download(phys=False)
+        for k in range(8):
+            assert same_bits(u[k], o.u0(k)), describe_mismatch(u[k], o.u0(k))
+
+
+def test_T_truncation_and_stop(mpd):
+    """The last step is cut to T and the loop stops at t == T (src/main.cpp:220-223)."""
+    text = cases.channel_config(30, 12, max_steps=0, T=0.0123456, dt=1e-3)
+    o, ores = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        res = sol.run()
+        assert res["steps"] == ores["steps"] and res["t"] == ores["t"] == 0.0123456
+        again = sol.run()
+        assert again["steps"] == 0
+        _compare_state(sol, o, "T-truncation")
+
+
+def test_convergence_loop(mpd):
+    """Convergence check every check_frequency steps, snapshot at init,
+    stop when SolutionChange < threshold (src/main.cpp:150-152, 233-247)."""
+    text = cases.channel_config(24, 10, max_steps=4000, threshold=2e-4, check_frequency=25)
+    o, ores = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        res = sol.run()
+        assert res["converged"] == ores["converged"]
+        assert res["steps"] == ores["steps"]
+        # The residual sums in a different order than the oracle: 1e-12 relative.
+        assert abs(res["change"] - ores["change"]) <= 1e-12 * abs(ores["change"])
+        _compare_state(sol, o, "convergence")
+
+
+def test_diagnostics(mpd):
+    text = cases.channel_config(64, 40, geometry="condiv", noise=1e-3, max_steps=30)
+    o, _ = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        sol.run()
+        assert sol.max_wave_speed() == o.max_wave_speed()
+        mf, th = sol.outlet()
+        assert mf == o.mass_flux() and th == o.thrust()
+        assert sol.radial_symmetry() == o.radial_symmetry()
+        # Conserved integrals: per-row tree sums vs the reference's serial sum.
+        ig, og = sol.integrals(), o.integrals()
+        assert np.all(np.abs(ig - og) <= 1e-13 * np.maximum(np.abs(og), 1e-300) + 1e-300), (ig, og)
+        o.save_prev()
+        g = sol.grid()
+        assert same_bits(g["r"], o.r) and same_bits(g["r_z"], o.r_z) and same_bits(g["dr"], o.dr)
+
+
+def test_solution_change_explicit(mpd):
+    text = cases.channel_config(32, 20, max_steps=0)
+    o = O.Oracle(cases.oracle_case(text))
+    o.save_prev()
+    with mpd.Solver(text) as sol:
+        first = sol.solution_change(save_prev=True)  # takes the IC snapshot
+        assert first == 0.0
+        for _ in range(10):
+            sol.advance(1e-4)
+            o.advance(1e-4)
+        c, co = sol.solution_change(save_prev=True), o.solution_change()
+        assert abs(c - co) <= 1e-12 * co
+
+
+def test_upload_download_roundtrip(mpd):
+    text = cases.channel_config(20, 10, max_steps=5)
+    with mpd.Solver(text) as sol:
+        u, p = sol.download()
+        u2 = [a.copy() for a in u]
+        sol.upload(u2, p)
+        ub, pb = sol.download()
+        for k in range(8):
+            assert same_bits(ub[k], u[k])
+        for k in range(10):
+            assert same_bits(pb[k], p[k])
+
+
+@pytest.mark.slow
+def test_straight_channel_2000_steps(mpd):
+    """The CPU-runnable workload in full: straight channel 200x80, 2000 steps,
+    all 8 fields within 1e-12 relative L-infinity of the oracle (and in fact
+    bit-identical)."""
+    text = cases.channel_config(200, 80, max_steps=2000)
+    o, ores = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        res = sol.run()
+        assert res["steps"] == 2000 and res["t"] == ores["t"]
+        u, _ = sol.download(phys=False)
+        for k in range(8):
+            assert rel_linf(u[k], o.u0(k)) <= 1e-12
+            assert same_bits(u[k], o.u0(k)), describe_mismatch(u[k], o.u0(k))
+
+
+@pytest.mark.slow
+def test_full_size_6400x2560(mpd):
+    """The memory-bound workload at full size: two steps bit-identical to the
+    oracle, and the tiled kernel identical to the simple one after 6 more."""
+    text = cases.channel_config(6400, 2560, geometry="condiv", noise=1e-3, max_steps=0)
+    o = O.Oracle(cases.oracle_case(text))
+    o.run(2)
+    with mpd.Solver(text) as sol:
+        sol.run(2)
+        u, _ = sol.download(phys=False)
+        for k in range(8):
+            assert same_bits(u[k], o.u0(k)), describe_mismatch(u[k], o.u0(k))
+        del o
+        sol.run(6)
+        ut, _ = sol.download(phys=False)
+    with mpd.Solver(text.replace("kernel: tiled", "kernel: simple")) as sol2:
+        sol2.run(8)
+        us, _ = sol2.download(phys=False)
+    for k in range(8):
+        assert same_bits(ut[k], us[k]), describe_mismatch(ut[k], us[k])
+        assert np.isfinite(ut[k][1:-1, 1:-1]).all()
