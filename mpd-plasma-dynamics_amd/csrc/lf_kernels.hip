@@ -549,7 +549,6 @@ __global__ void __launch_bounds__(256) lf_bc_ring(StepArgs a, double dt_explicit
             if (face_covers(bc, MPD_M_HI, l)) f = MPD_M_HI;
         }
         double u[8];
-        bool have = true;
         if (f >= 0) {
             bc_final(a, f, l, m, dt, u);
         } else if (m >= 2 && m <= Mn - 1) {
@@ -557,12 +556,11 @@ __global__ void __launch_bounds__(256) lf_bc_ring(StepArgs a, double dt_explicit
             const double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
             for (int k = 0; k < 8; ++k) u[k] = rb[(int64_t)k * a.pitch + m];
         } else {
-            // Uncovered column cell: the reference never writes u there, so it
-            // stays 0 (zero-initialised); the buffers already hold 0.
-            have = false;
+            // Uncovered column cell: the reference never writes u there, so u
+            // (and u0 after the copy) stays 0 whatever the start state held.
             for (int k = 0; k < 8; ++k) u[k] = 0.0;
         }
-        if (have) {
+        {
             const int64_t i = at(a, l, m);
             for (int k = 0; k < 8; ++k) a.u_out[(int64_t)k * a.fstride + i] = u[k];
         }
