@@ -1,0 +1,201 @@
+"""Host-side tests (CPU only): the C ABI surface, the config grammar and its
+errors (src/config.cpp:52-384, src/bcs/per_field_bc.cpp:188-243), the YAML
+reader, the expression compiler, and the initial state against the oracle."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import cases
+from parity import same_bits
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(mpd):
+    hdr = open(os.path.join(REPO, "include", "mpd_lf.h")).read()
+    declared = set(re.findall(r"\b(mpd_[a-z_0-9]+)\s*\(", hdr))
+    assert declared == set(mpd.EXPORTS), declared ^ set(mpd.EXPORTS)
+    lib = mpd.load()
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+BASE = cases.channel_config(12, 6, max_steps=5)
+
+
+def _with_bc(face_yaml):
+    i = BASE.index("boundary_conditions:")
+    j = BASE.index("gpu:")
+    return BASE[:i] + "boundary_conditions:\n" + face_yaml + BASE[j:]
+
+
+@pytest.mark.parametrize("face,msg", [
+    ("  l_lo:\n    - v_r: wall_tangent\n", "no longer a built-in preset"),
+    ("  l_lo:\n    - H_phi: hphi_r0_over_r\n", "no longer a built-in preset"),
+    ("  l_lo:\n    - v_r: { type: wall_tangent }\n", "no longer supported"),
+    ("  l_lo:\n    - v_r: { type: bogus }\n", "unknown field condition type"),
+    ("  l_lo:\n    - v_r: { foo: 1 }\n", "must contain one of"),
+    ("  l_lo:\n    - v_r: { type: expression }\n", "requires an 'expr' key"),
+    ("  l_lo:\n    - rho: axis_lf\n", "only valid on the M_LO face"),
+    ("  m_lo:\n    - v_r: axis_lf\n", "has no AxisLF formula"),
+    ("  m_lo:\n    - range: [1, 2, 3]\n", "two-element sequence"),
+    ("  m_lo:\n    - v_r: \"v_z * (\"\n", "failed to compile BC expression"),
+    ("  m_lo:\n    - v_r: \"nope * 2\"\n", "unknown symbol"),
+    ("  m_lo:\n    - rho: \"e * 2\"\n", "forward reference"),
+    ("  m_lo:\n    - v_r: \"v_r_nb + v_r\"\n", "forward reference"),
+    ("  m_lo: { a: 1 }\n", "must be a YAML sequence"),
+])
+def test_bc_grammar_errors(mpd, face, msg):
+    with pytest.raises(mpd.MpdError) as ei:
+        mpd.config_check(_with_bc(face))
+    assert msg in str(ei.value)
+    assert ei.value.code == mpd.MPD_E_CONFIG
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("initial_conditions:\n  type: foo\ngrid: { L_max: 5, M_max: 4 }\n", "no longer supported"),
+    ("initial_conditions:\n  params: {}\ngrid: { L_max: 5, M_max: 4 }\n", "no longer supported"),
+    ("initial_conditions:\n  vars: { rho: 1 }\ngrid: { L_max: 5, M_max: 4 }\n", "shadows the built-in"),
+    ("initial_conditions:\n  rho: { a: 1 }\ngrid: { L_max: 5, M_max: 4 }\n", "must be a scalar"),
+    ("geometry: { type: nonexistent }\ngrid: { L_max: 5, M_max: 4 }\n", "unknown geometry type"),
+    ("grid: { L_max: 2, M_max: 4 }\n", "L_max must be"),
+    ("grid: { L_max: 5, M_max: 4 }\nadaptive_dt: { enabled: maybe }\n", "bool"),
+    ("grid:\n  L_max: 5\n   M_max: 4\n", "YAML"),
+])
+def test_config_errors(mpd, text, msg):
+    with pytest.raises(mpd.MpdError) as ei:
+        mpd.config_check(text)
+    assert msg in str(ei.value)
+
+
+def test_bc_grammar_forms_accepted(mpd):
+    face = """  l_lo:
+    - rho: neumann
+      v_z: { neumann: 0.5 }
+      v_r: dirichlet
+      v_phi: { dirichlet: 1.0 }
+      H_z: "H_z0"
+      H_r: { expr: "v_z_nb * r_z_nb" }
+      H_phi: { type: expression, expr: "r_0 / r" }
+      e: { type: neumann, gradient: 0.25 }
+  l_hi:
+    - rho: { type: neumann, value: 0.1 }
+      v_z: { type: dirichlet }
+      v_r: { type: dirichlet, value: 2.0 }
+      e: { type: expr, expr: "e_nb" }
+  m_lo:
+    - range: [-1, 3]
+      rho: { type: axis_lf }
+      v_z: axis_lf
+    - range: [4, -1]
+  m_hi:
+    -
+      v_r: "v_z * r_z"
+"""
+    assert mpd.config_check(_with_bc(face)) == (12, 6)
+
+
+def test_yaml_reader_features(mpd):
+    text = """
+# comment line
+physics:   # trailing comment
+  gamma: 1.4     # comment after value
+  beta: '0.1'
+time: {T: 2.5, dt: "1.0e-4"}
+grid:
+  L_max: 7
+  M_max: 5
+geometry:
+  type: "straight_coaxial"
+  params: { r_inner: 0.25, r_outer: 0.75 }
+initial_conditions:
+  H_phi: "0.1 # not a comment / r"
+boundary_conditions:
+  l_lo:
+  - rho: neumann
+  - range: [ 2 , 4 ]
+    v_z: {dirichlet: 3}
+"""
+    # '#' inside quotes is text: the H_phi expression is invalid, which proves
+    # the comment was not stripped.
+    with pytest.raises(mpd.MpdError, match="unexpected"):
+        mpd.config_check(text)
+    assert mpd.config_check(text.replace("0.1 # not a comment / r", "0.1 / r")) == (7, 5)
+
+
+@pytest.mark.parametrize("name,text", [
+    ("straight", cases.channel_config(20, 9)),
+    ("condiv_noise", cases.channel_config(17, 11, geometry="condiv", noise=1e-3, seed=3)),
+    ("thruster", cases.thruster_config(40, 20)),
+    ("mixed", cases.mixed_config(15, 8)),
+])
+def test_initial_state_matches_oracle(mpd, name, text):
+    u, p = mpd.initial_state(text)
+    o = O.Oracle(cases.oracle_case(text))
+    for k in range(10):
+        assert same_bits(p[k], o.phys(k)), O.PHYS_NAMES[k]
+    for k in range(8):
+        assert same_bits(u[k], o.u0(k)), k
+
+
+def _ic_value(mpd, expr, field="rho", extra=""):
+    """Value of an IC expression at cell (1, 1) of a straight channel."""
+    text = f"""
+physics: {{ gamma: 1.4, beta: 0.1, H_z0: 0.3 }}
+grid: {{ L_max: 4, M_max: 3 }}
+geometry: {{ type: straight_coaxial, params: {{ r_inner: 0.2, r_outer: 0.8 }} }}
+initial_conditions:
+{extra}  {field}: "{expr}"
+"""
+    _, p = mpd.initial_state(text)
+    idx = {"rho": O.RHO, "v_z": O.VZ, "e": O.E, "H_phi": O.HPHI}[field]
+    return p[idx][1, 1]
+
+
+@pytest.mark.parametrize("expr,expected", [
+    ("0.2 / r", 0.2 / 0.2),
+    ("1 / 3", 1.0 / 3.0),
+    ("0.5 / ((gamma - 1) * 1.7)", 0.5 / ((1.4 - 1.0) * 1.7)),
+    ("2 ^ 3 ^ 2", 512.0),
+    ("-2 ^ 2", -4.0),
+    ("2 ^ -1", 0.5),
+    ("(1 - 0.9 * z) * r_0 / r", (1 - 0.9 * 0.0) * 0.5 / 0.2),
+    ("min(3, 1, 2) + max(4, 9) * 10", 91.0),
+    ("clamp(0, 7, 5)", 5.0),
+    ("abs(-2.5) + sign(-3) + sign(0)", 1.5),
+    ("pi", math.pi),
+    ("sqrt(2)", math.sqrt(2.0)),
+    ("exp(1)", math.exp(1.0)),
+    ("atan2(1, 2)", math.atan2(1.0, 2.0)),
+    ("pow(2, 0.5)", 2.0 ** 0.5),
+    ("log(10) + log10(1000) + log2(8)", math.log(10.0) + 3.0 + 3.0),
+    ("7 % 3", 1.0),
+    ("+3 - -2", 5.0),
+    ("H_z0 * beta", 0.3 * 0.1),
+])
+def test_expression_known_answers(mpd, expr, expected):
+    assert _ic_value(mpd, expr) == expected
+
+
+def test_ic_user_vars_and_forward_reference(mpd):
+    assert _ic_value(mpd, "amp * 2", extra="  vars: { amp: 0.25 }\n") == 0.5
+    # In the IC a forward reference reads the previous cell's value (0 at the
+    # first cell), as exprtk's persistent symbols do.
+    assert _ic_value(mpd, "e + 1", extra="  e: \"5\"\n") == 1.0
+
+
+def test_initial_state_full_size_matches_oracle(mpd):
+    """The converging-diverging 6400x2560 IC: sin(pi z) of the wall must be
+    glibc's sin in both codes (a fused sincos rounds differently on ~1e-5 of
+    the rows' arguments, which once made 569 cells of H_phi = 0.2/r differ)."""
+    text = cases.channel_config(6400, 2560, geometry="condiv", noise=1e-3, max_steps=0)
+    u, p = mpd.initial_state(text)
+    o = O.Oracle(cases.oracle_case(text))
+    for k in range(10):
+        assert same_bits(p[k], o.phys(k)), O.PHYS_NAMES[k]
+    for k in range(8):
+        assert same_bits(u[k], o.u0(k)), k
