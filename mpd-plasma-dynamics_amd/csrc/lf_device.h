@@ -26,6 +26,7 @@ struct StepArgs {
     const double* drin;   // d r_inner / dz
     const double* drout;  // d r_outer / dz
     const double* dr;     // dr[l] = (r_outer - r_inner) / M_max
+    const double* dr2inv; // RN(1 / (2 dr[l])), the divisor's reciprocal for div_const
     double gamma, gm1, beta, H_z0, r_0, dz, dy, dz2;
     double cfl_dx, growth, dt_min, dt_max, T;
     int adaptive;
@@ -111,19 +112,21 @@ __device__ __forceinline__ double wave_speed_ph(const double* ph, double gamma) 
 }
 
 // Correctly rounded x / c for a positive normal constant c with cinv =
-// RN(1/c): a product estimate refined by two residual corrections; the last
-// correction starts within one ulp, the case in which Markstein's theorem
-// makes RN(q + r * cinv) with the exact residual r = x - q * c equal to
+// RN(1/c): a product estimate refined by two residual corrections.  The
+// first correction makes the estimate faithful (within one ulp); from a
+// faithful estimate the residual x - q c is exact and RN(q + r cinv) equals
 // RN(x / c) (Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al.,
-// Handbook of Floating-Point Arithmetic, 2nd ed., sec. 4.9).  The sign of a
-// zero quotient is restored from x.  Valid for normal, finite x.
+// Handbook of Floating-Point Arithmetic, 2nd ed., sec. 4.9).  The residual
+// is carried negated, e = q c - x, so that a zero x keeps its sign through
+// the corrections (-0 / c = -0) without a separate copysign.  Valid for
+// finite x whose quotient is normal; an infinite x gives NaN where IEEE
+// division gives infinity.
 __device__ __forceinline__ double div_const(double x, double c, double cinv) {
     const double q0 = x * cinv;
-    const double r0 = __builtin_fma(-q0, c, x);
-    const double q1 = __builtin_fma(r0, cinv, q0);
-    const double r1 = __builtin_fma(-q1, c, x);
-    const double q2 = __builtin_fma(r1, cinv, q1);
-    return __builtin_copysign(q2, x);
+    const double e0 = __builtin_fma(q0, c, -x);
+    const double q1 = __builtin_fma(-e0, cinv, q0);
+    const double e1 = __builtin_fma(q1, c, -x);
+    return __builtin_fma(-e1, cinv, q1);
 }
 
 struct DivExact {

@@ -129,9 +129,9 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
     }
 }
 
-// Record of row l at column m for the marching kernel; zero physical values on
-// the ghost rows and beyond the last interior column.
-__device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, bool mload, const double* raw, Rec& c) {
+// Record of row l for the marching kernel from its 8 conserved values; zero
+// physical values on the ghost rows and beyond the last interior column.
+__device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, const double* raw, Rec& c) {
     for (int k = 0; k < 8; ++k) c.u[k] = raw[k];
     c.r = cell_r(a, l, m);
     if (l == 0 || l == a.L + 1 || m > a.Mn) {
@@ -139,17 +139,60 @@ __device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, bool ml
     } else {
         rec_phys(c, a.gm1);
     }
-    (void)mload;
 }
 
-__device__ __forceinline__ void load_raw(const StepArgs& a, int l, int m, bool mload, double* raw) {
-    if (mload) {
-        const int64_t i = at(a, l, m);
-        for (int k = 0; k < 8; ++k) raw[k] = fld(a.u_in, a.fstride, k)[i];
-    } else {
-        for (int k = 0; k < 8; ++k) raw[k] = 0.0;
+// State-buffer access of the marching kernel.  With kBuf, raw buffer
+// instructions: a scalar descriptor for the buffer, the row and field offset
+// in a scalar register and this lane's column offset in a 32-bit vector
+// register, so no per-lane 64-bit address arithmetic (the whole 8-field
+// buffer must stay below 2^31 bytes).  Without kBuf, plain pointers.
+// Lanes past the ghost column address the ghost column, which holds 0 in
+// every state buffer (never written).
+template <bool kBuf>
+struct StateIO {
+    __amdgpu_buffer_rsrc_t rin, rout;
+    const double* pin;
+    double* pout;
+    int64_t row_bytes, field_bytes;
+    uint32_t moff;
+
+    __device__ __forceinline__ void init(const StepArgs& a, uint32_t moff_) {
+        moff = moff_;
+        row_bytes = (int64_t)a.pitch * 8;
+        field_bytes = a.fstride * 8;
+        pin = a.u_in;
+        pout = a.u_out;
+        if (kBuf) {
+            rin = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(a.u_in), 0, -1, 0x00020000);
+            rout = __builtin_amdgcn_make_buffer_rsrc(a.u_out, 0, -1, 0x00020000);
+        }
     }
-}
+    __device__ __forceinline__ void load_row(int l, double* raw) const {
+        if (kBuf) {
+            const int so = (int)(l * row_bytes);
+            for (int k = 0; k < 8; ++k) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rin, moff, so + (int)(k * field_bytes), 0);
+                raw[k] = __builtin_bit_cast(double, v);
+            }
+        } else {
+            const char* row = reinterpret_cast<const char*>(pin) + l * row_bytes + moff;
+            for (int k = 0; k < 8; ++k) raw[k] = *reinterpret_cast<const double*>(row + k * field_bytes);
+        }
+    }
+    __device__ __forceinline__ void store_row(int l, const double* o) const {
+        if (kBuf) {
+            const int so = (int)(l * row_bytes);
+            using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(rin, 0, 0, 0));
+            for (int k = 0; k < 8; ++k)
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, o[k]), rout, moff,
+                                                      so + (int)(k * field_bytes), 0);
+        } else {
+            char* row = reinterpret_cast<char*>(pout) + l * row_bytes + moff;
+            for (int k = 0; k < 8; ++k)
+                __builtin_nontemporal_store(o[k], reinterpret_cast<double*>(row + k * field_bytes));
+        }
+    }
+};
 
 __device__ __forceinline__ void shfl_rec(const Rec& c, Rec& e, Rec& w) {
     for (int k = 0; k < 8; ++k) {
@@ -166,67 +209,130 @@ __device__ __forceinline__ void shfl_rec(const Rec& c, Rec& e, Rec& w) {
     e.r = __shfl_down(c.r, 1, 64);       w.r = __shfl_up(c.r, 1, 64);
 }
 
-// The production stencil.  Each wavefront owns a tile of 64 columns (62
-// outputs plus one halo column on each side) by H rows and marches down the
-// rows keeping the records of rows l-1, l, l+1 in registers: every cell's
-// conserved values are read from HBM once and its physical values computed
-// once; the m +- 1 neighbours come from the adjacent lanes (ds_bpermute),
-// the l +- 1 neighbours from the register window; the next row's loads are
-// issued before the current row's arithmetic.  Tiles are ordered
-// column-strip-fastest so neighbouring waves share halo columns in L2.
-template <bool kSpeed, class Div>
-__global__ void __launch_bounds__(256) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
-                                                        int nstrips, int ntiles) {
-    if (loop && a.st->done) return;
-    const int lane = (int)(threadIdx.x & 63);
-    const int tile = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-    if (tile >= ntiles) return;
-    const int strip = tile % nstrips, chunk = tile / nstrips;
-    const int m = 1 + 62 * strip + lane;
-    const int l0 = 1 + chunk * H;
-    const int l1 = min(l0 + H, a.L + 1);
-    const double dt = step_dt(a, dt_explicit, loop);
-    const bool mload = m <= a.Mn + 1;
-    const bool outlane = lane >= 1 && lane <= 62 && m <= a.Mn - 1;
-    const double dz2 = a.dz2, dz2i = 1.0 / a.dz2;
-    Rec S, C, N;
-    {
-        double raw[8];
-        load_raw(a, l0 - 1, m, mload, raw);
-        row_rec(a, l0 - 1, m, mload, raw, S);
-        load_raw(a, l0, m, mload, raw);
-        row_rec(a, l0, m, mload, raw, C);
-        load_raw(a, l0 + 1, m, mload, raw);
-        row_rec(a, l0 + 1, m, mload, raw, N);
-    }
-    uint64_t key = 0;
-    for (int l = l0; l < l1; ++l) {
-        const bool pre = (l + 2 <= l1);
-        double raw[8];
-        if (pre) load_raw(a, l + 2, m, mload, raw);
-        Rec E, W;
-        shfl_rec(C, E, W);
-        const double dr2 = 2 * a.dr[l];
-        const double dr2i = 1.0 / dr2;
-        double o[8];
-        lf_cell<Div>(N, S, E, W, C, dt, dz2, dz2i, dr2, dr2i, o);
-        if (outlane) {
-            if (l == 1 || l == a.L) {
-                double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
-                for (int k = 0; k < 8; ++k) rb[(int64_t)k * a.pitch + m] = o[k];
-            } else {
-                const int64_t i = at(a, l, m);
-                for (int k = 0; k < 8; ++k) __builtin_nontemporal_store(o[k], a.u_out + (int64_t)k * a.fstride + i);
-                if (kSpeed) {
+// Single-precision estimate of the wave speed v + c_s + c_a of conserved
+// values u at radius r (gg1 = gamma (gamma - 1), so c_s^2 = gg1 e).  Its
+// relative error is a few 2^-23 (hardware reciprocal and square root, one
+// ulp each); SPEED_EST_TOL bounds it with a wide margin.  NaN or infinity
+// whenever an input is non-finite or out of single range.
+__device__ __forceinline__ float speed_estimate(const double* u, double r, float gg1) {
+    const float fu0 = (float)u[0], fr = (float)r;
+    const float iu = __builtin_amdgcn_rcpf(fu0);
+    const float ir = __builtin_amdgcn_rcpf(fr);
+    const float vz = (float)u[1] * iu, vr = (float)u[2] * iu, e = (float)u[4] * iu;
+    const float hz = (float)u[6] * ir, hr = (float)u[7] * ir, hp = (float)u[5];
+    const float cs = __builtin_amdgcn_sqrtf(gg1 * e);
+    const float ca = __builtin_amdgcn_sqrtf((hz * hz + hr * hr + hp * hp) * (fr * iu));
+    const float v = __builtin_amdgcn_sqrtf(vz * vz + vr * vr);
+    return v + cs + ca;
+}
+
+// Threshold of the exact wave-speed evaluation in the loop: only cells whose
+// estimate reaches SPEED_CAND_FRAC of T = (1 - SPEED_DROP) * (last step's
+// max) are evaluated exactly.  A cell left out has exact speed below
+// SPEED_CAND_FRAC T / (1 - SPEED_EST_TOL) < T, so when the exact max of the
+// evaluated cells is >= T it is the max over all cells; otherwise (the max
+// fell by more than SPEED_DROP in one step) lf_speed_fallback rescans.
+#define SPEED_DROP 1.0e-3
+#define SPEED_CAND_FRAC 0.9999f
+#define SPEED_EST_TOL 1.0e-5
+
+__device__ __forceinline__ double speed_T(const StepArgs& a, int loop) {
+    if (!loop) return 0.0;
+    const double T = a.st->max_speed * (1.0 - SPEED_DROP);
+    return (T > 1.0e-30) ? T : 0.0;  // also 0 for NaN
+}
+
+struct TileCtx {
+    int m, l1;
+    uint32_t moff;
+    bool outlane;
+    double dt, dz2, dz2i;
+    float thr, gg1;
+};
+
+// One row of the march: output row l from the records S (l-1), C (l), N
+// (l+1); the loads of row l+2 are issued first and land in S, which the
+// caller rotates to the N slot.
+template <bool kSpeed, bool kBuf, class Div>
+__device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, const StateIO<kBuf>& io, int l,
+                                          Rec& S, const Rec& C, const Rec& N, uint64_t& key) {
+    const bool pre = (l + 2 <= x.l1);
+    double raw[8];
+    if (pre) io.load_row(l + 2, raw);
+    Rec E, W;
+    shfl_rec(C, E, W);
+    const double dr2 = 2 * a.dr[l];
+    double o[8];
+    lf_cell<Div>(N, S, E, W, C, x.dt, x.dz2, x.dz2i, dr2, a.dr2inv[l], o);
+    if (x.outlane) {
+        if (l == 1 || l == a.L) {
+            char* rb = reinterpret_cast<char*>(a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch) + x.moff;
+            for (int k = 0; k < 8; ++k) *reinterpret_cast<double*>(rb + (int64_t)k * a.pitch * 8) = o[k];
+        } else {
+            io.store_row(l, o);
+            if (kSpeed) {
+                const float est = speed_estimate(o, C.r, x.gg1);
+                if (!(est < x.thr)) {
                     const uint64_t kk = speed_key(wave_speed_u(o, C.r, a.gamma, a.gm1));
                     key = (kk > key) ? kk : key;
                 }
             }
         }
-        S = C;
-        C = N;
-        if (pre) row_rec(a, l + 2, m, mload, raw, N);
     }
+    if (pre) row_rec(a, l + 2, x.m, raw, S);
+}
+
+// The production stencil.  Each wavefront owns a tile of 64 columns (62
+// outputs plus one halo column on each side) by H rows and marches down the
+// rows with the records of rows l-1, l, l+1 in registers (three slots used
+// round-robin, so nothing is copied between rows): every cell's conserved
+// values are read from HBM once and its physical values computed once; the
+// m +- 1 neighbours come from the adjacent lanes (ds_bpermute), the l +- 1
+// neighbours from the register window; each row's loads are issued a row
+// ahead of their use.  Tiles are ordered column-strip-fastest so that
+// neighbouring waves share halo columns in L2.
+template <bool kSpeed, bool kBuf, class Div>
+__global__ void __launch_bounds__(256) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
+                                                        int nstrips, int ntiles) {
+    if (loop && a.st->done) return;
+    const int lane = (int)(threadIdx.x & 63);
+    // Wave-uniform tile index (scalar registers: row addressing and the march
+    // loop then run on the scalar unit).
+    const int tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (tile >= ntiles) return;
+    const int strip = tile % nstrips, chunk = tile / nstrips;
+    TileCtx x;
+    x.m = 1 + 62 * strip + lane;
+    const int l0 = 1 + chunk * H;
+    x.l1 = min(l0 + H, a.L + 1);
+    x.dt = step_dt(a, dt_explicit, loop);
+    x.moff = (uint32_t)(min(x.m, a.Mn + 1)) * 8u;
+    x.outlane = lane >= 1 && lane <= 62 && x.m <= a.Mn - 1;
+    x.dz2 = a.dz2;
+    x.dz2i = 1.0 / a.dz2;
+    x.thr = kSpeed ? (float)speed_T(a, loop) * SPEED_CAND_FRAC : 0.0f;
+    x.gg1 = (float)(a.gamma * a.gm1);
+    StateIO<kBuf> io;
+    io.init(a, x.moff);
+    Rec A, B, C;
+    {
+        double raw[8];
+        io.load_row(l0 - 1, raw);
+        row_rec(a, l0 - 1, x.m, raw, A);
+        io.load_row(l0, raw);
+        row_rec(a, l0, x.m, raw, B);
+        io.load_row(l0 + 1, raw);
+        row_rec(a, l0 + 1, x.m, raw, C);
+    }
+    uint64_t key = 0;
+    int l = l0;
+    for (; l + 3 <= x.l1; l += 3) {
+        march_row<kSpeed, kBuf, Div>(a, x, io, l, A, B, C, key);
+        march_row<kSpeed, kBuf, Div>(a, x, io, l + 1, B, C, A, key);
+        march_row<kSpeed, kBuf, Div>(a, x, io, l + 2, C, A, B, key);
+    }
+    if (l < x.l1) march_row<kSpeed, kBuf, Div>(a, x, io, l, A, B, C, key);
+    if (l + 1 < x.l1) march_row<kSpeed, kBuf, Div>(a, x, io, l + 1, B, C, A, key);
     if (kSpeed) {
         key = wave_max_u64(key);
         if (lane == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
@@ -582,6 +688,47 @@ __global__ void __launch_bounds__(256) lf_bc_ring(StepArgs a, double dt_explicit
     __syncthreads();
     if (last && threadIdx.x == 0) {
         __threadfence();
+        // The interior max came from the cells whose estimate passed the
+        // threshold; it is the max over all cells only if it reaches T.
+        const double T = speed_T(a, loop);
+        const uint64_t bits = __hip_atomic_load(&a.st->speed_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double mc = __longlong_as_double((long long)bits);
+        if (speed && T > 0.0 && !(mc >= T)) {
+            a.st->fallback = 1;
+            a.st->ticket = 0;
+        } else {
+            finalize_step(a);
+        }
+    }
+}
+
+// Exact max wave speed over the whole new state, run when the candidate max
+// of the step could not be proven to be the max (see SPEED_DROP); then the
+// step's dt bookkeeping.  Exits at once otherwise.
+__global__ void __launch_bounds__(256) lf_speed_fallback(StepArgs a) {
+    if (!a.st->fallback) return;
+    const int64_t n = (int64_t)a.L * a.Mn;
+    uint64_t key = 0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int l = 1 + (int)(c / a.Mn), m = 1 + (int)(c % a.Mn);
+        double u[8];
+        for (int k = 0; k < 8; ++k) u[k] = a.u_out[(int64_t)k * a.fstride + at(a, l, m)];
+        const uint64_t kk = speed_key(wave_speed_u(u, cell_r(a, l, m), a.gamma, a.gm1));
+        key = (kk > key) ? kk : key;
+    }
+    key = wave_max_u64(key);
+    if ((threadIdx.x & 63) == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(&a.st->ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        a.st->fallback = 0;
         finalize_step(a);
     }
 }
@@ -804,8 +951,15 @@ hipError_t launch_tiled(const StepArgs& a, int H, double dt, int loop, int speed
     const int nchunks = (a.L + H - 1) / H;
     const int ntiles = nstrips * nchunks;
     const unsigned blocks = (unsigned)((ntiles + 3) / 4);
-    if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
-    else hipLaunchKernelGGL((lf_stencil_tiled<false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+    // Buffer addressing needs every row/field offset of a state below 2^31 bytes.
+    const bool buf = (double)a.fstride * 8.0 * 8.0 < 2147483648.0;
+    if (buf) {
+        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+        else hipLaunchKernelGGL((lf_stencil_tiled<false, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+    } else {
+        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+        else hipLaunchKernelGGL((lf_stencil_tiled<false, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+    }
     return hipGetLastError();
 }
 
@@ -842,6 +996,11 @@ int residual_blocks(const StepArgs& a) {
     const long long n = (long long)a.L * a.Mn;
     long long b = (n + 255) / 256;
     return (int)(b < 1024 ? b : 1024);
+}
+
+hipError_t launch_speed_fallback(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(lf_speed_fallback, dim3(256), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_residual(const StepArgs& a, double* prev, double* partial, int loop, int check_frequency,

@@ -21,6 +21,9 @@ hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt_expl
                           hipStream_t s);
 // Boundary faces; with loop the last block finishes the step's loop state.
 hipError_t launch_bc(const StepArgs& a, double dt_explicit, int loop, int speed, hipStream_t s);
+// Exact max-speed rescan + dt bookkeeping when the boundary kernel flagged
+// that the step's candidate max is not provably the max (loop only).
+hipError_t launch_speed_fallback(const StepArgs& a, hipStream_t s);
 // SolutionChange (+ SavePrev when save).  loop: run only on a check step.
 hipError_t launch_residual(const StepArgs& a, double* prev, double* partial, int loop, int check_frequency,
                            double threshold, int save, hipStream_t s);

@@ -82,5 +82,5 @@ struct MpdLoopState {
     uint32_t ticket;      // completion counter for last-block finalisation
     uint32_t ticket2;
     int32_t checked_step;  // step at which SolutionChange last ran in the loop
-    int32_t pad;
+    int32_t fallback;      // 1: the candidate max is not provably the max; rescan
 };

@@ -232,6 +232,7 @@ struct mpd_solver {
         a.drin = geo + 2 * rows;
         a.drout = geo + 3 * rows;
         a.dr = geo + 4 * rows;
+        a.dr2inv = geo + 5 * rows;
         a.gamma = cfg.gamma;
         a.gm1 = cfg.gamma - 1.0;
         a.beta = cfg.beta;
@@ -305,14 +306,15 @@ struct mpd_solver {
         partial = Alloc(2 * 1024);
         rowsum = Alloc((size_t)7 * rows);
         scratch = Alloc(64);
-        geo = Alloc((size_t)5 * rows);
-        std::vector<double> g((size_t)5 * rows);
+        geo = Alloc((size_t)6 * rows);
+        std::vector<double> g((size_t)6 * rows);
         for (int l = 0; l < rows; ++l) {
             g[l] = grid.rin[l];
             g[rows + l] = grid.rout[l];
             g[2 * rows + l] = grid.drin[l];
             g[3 * rows + l] = grid.drout[l];
             g[4 * rows + l] = grid.dr[l];
+            g[5 * rows + l] = 1.0 / (2 * grid.dr[l]);
         }
         HIPCHK(hipMemcpy(geo, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMalloc((void**)&bc, sizeof(MpdBcProgram)));
@@ -417,6 +419,7 @@ struct mpd_solver {
         const StepArgs a = Args(in);
         HIPCHK(mpd::launch_stencil(a, scfg, 0.0, 1, 1, stream));
         HIPCHK(mpd::launch_bc(a, 0.0, 1, 1, stream));
+        HIPCHK(mpd::launch_speed_fallback(a, stream));
         if (conv) {
             StepArgs r = Args(1 - in);
             r.step1 = 0;
@@ -459,7 +462,7 @@ struct mpd_solver {
         h.dt = dt;
         h.dt_use = (t + dt > cfg.T) ? cfg.T - t : dt;
         h.change = change;
-        h.max_speed = speed;
+        h.max_speed = (speed_src == 2) ? speed : 0.0;  // the stencil's candidate threshold
         h.step = step;
         h.max_steps = (cap == LLONG_MAX) ? 0 : (int)std::min(cap, (long long)INT_MAX);
         h.converged = converged ? 1 : 0;
@@ -554,6 +557,7 @@ struct mpd_solver {
                 HIPCHK(mpd::launch_stencil(a, scfg, 0.0, 1, 1, stream));
                 HIPCHK(hipEventRecord(ev[3 * i + 1], stream));
                 HIPCHK(mpd::launch_bc(a, 0.0, 1, 1, stream));
+                HIPCHK(mpd::launch_speed_fallback(a, stream));
                 HIPCHK(hipEventRecord(ev[3 * i + 2], stream));
                 if (conv)
                     HIPCHK(mpd::launch_residual(Args(1 - in), prev, partial, 1, cfg.check_frequency,
