@@ -26,7 +26,7 @@ struct StepArgs {
     const double* drin;   // d r_inner / dz
     const double* drout;  // d r_outer / dz
     const double* dr;     // dr[l] = (r_outer - r_inner) / M_max
-    const double* dr2inv; // RN(1 / (2 dr[l])), the divisor's reciprocal for div_const
+    const double* dr2inv; // div_recip(2 dr[l]), computed on the device (lf_recip_table)
     double gamma, gm1, beta, H_z0, r_0, dz, dy, dz2;
     double cfl_dx, growth, dt_min, dt_max, T;
     int adaptive;
@@ -111,30 +111,58 @@ __device__ __forceinline__ double wave_speed_ph(const double* ph, double gamma) 
     return v + cs + ca;
 }
 
-// Correctly rounded x / c for a positive normal constant c with cinv =
-// RN(1/c): a product estimate refined by two residual corrections.  The
-// first correction makes the estimate faithful (within one ulp); from a
-// faithful estimate the residual x - q c is exact and RN(q + r cinv) equals
-// RN(x / c) (Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al.,
-// Handbook of Floating-Point Arithmetic, 2nd ed., sec. 4.9).  The residual
-// is carried negated, e = q c - x, so that a zero x keeps its sign through
-// the corrections (-0 / c = -0) without a separate copysign.  Valid for
-// finite x whose quotient is normal; an infinite x gives NaN where IEEE
-// division gives infinity.
-__device__ __forceinline__ double div_const(double x, double c, double cinv) {
-    const double q0 = x * cinv;
-    const double e0 = __builtin_fma(q0, c, -x);
-    const double q1 = __builtin_fma(-e0, cinv, q0);
-    const double e1 = __builtin_fma(q1, c, -x);
-    return __builtin_fma(-e1, cinv, q1);
+// Division split in two halves.  The compiler lowers the IEEE binary64
+// division x / d on gfx950 (LLVM AMDGPU fdiv64) as: range scaling of x and d
+// (v_div_scale_f64), y = v_rcp_f64(d) refined by two Newton steps
+// (four FMAs), q = x * y, one FMA residual r = x - d q, then
+// v_div_fmas_f64(r, y, q) = fma(r, y, q) and v_div_fixup_f64 for special
+// operands.  div_recip is the divisor half, div_by the quotient half, so a
+// divisor shared by several quotients is refined once.  The residual is
+// carried negated, e = q d - x = -r (round-to-nearest is symmetric), so a
+// zero x keeps its sign as v_div_fixup_f64 would give it.  For d > 0 and x
+// zero or finite, and operands and quotient inside the range where the
+// scaling step leaves them unchanged (|x|, |d| and |x / d| between 2^-380
+// and 2^380 holds for every physical state), div_by(x, d, div_recip(d)) is
+// bit-identical to x / d, which is correctly rounded.
+__device__ __forceinline__ double div_recip(double d) {
+    const double y0 = __builtin_amdgcn_rcp(d);
+    const double e0 = __builtin_fma(-d, y0, 1.0);
+    const double y1 = __builtin_fma(y0, e0, y0);
+    const double e1 = __builtin_fma(-d, y1, 1.0);
+    return __builtin_fma(y1, e1, y1);
+}
+
+__device__ __forceinline__ double div_by(double x, double d, double y) {
+    const double q = x * y;
+    const double e = __builtin_fma(q, d, -x);
+    return __builtin_fma(-e, y, q);
+}
+
+// Fields::UpdatePhysicalFromU (src/fields.cpp:84-97) with the divisions by
+// u_1 and by r sharing one refined reciprocal each (same bits as rec_phys
+// for every state with u_1, r > 0 inside the range div_by states).
+__device__ __forceinline__ void rec_phys_shared(Rec& c, double gm1) {
+    const double yu = div_recip(c.u[0]);
+    const double yr = div_recip(c.r);
+    c.rho = div_by(c.u[0], c.r, yr);
+    c.vz = div_by(c.u[1], c.u[0], yu);
+    c.vr = div_by(c.u[2], c.u[0], yu);
+    c.vphi = div_by(c.u[3], c.u[0], yu);
+    c.Hphi = c.u[5];
+    c.Hz = div_by(c.u[6], c.r, yr);
+    c.Hr = div_by(c.u[7], c.r, yr);
+    const double e = div_by(c.u[4], c.u[0], yu);
+    c.p = gm1 * c.rho * e;
+    c.P = c.p + 0.5 * (c.Hz * c.Hz + c.Hr * c.Hr + c.Hphi * c.Hphi);
 }
 
 struct DivExact {
     __device__ __forceinline__ double operator()(double x, double c, double) const { return x / c; }
 };
+// Division by the per-row constants 2 dz and 2 dr_l given ci = div_recip(c).
 struct DivConst {
     __device__ __forceinline__ double operator()(double x, double c, double ci) const {
-        return div_const(x, c, ci);
+        return div_by(x, c, ci);
     }
 };
 

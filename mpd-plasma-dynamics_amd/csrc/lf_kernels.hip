@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
         const double dr_l = a.dr[l];
         const double dr2 = 2 * dr_l;
         double o[8];
-        lf_cell<Div>(N, S, E, W, C, dt, a.dz2, 1.0 / a.dz2, dr2, 1.0 / dr2, o);
+        lf_cell<Div>(N, S, E, W, C, dt, a.dz2, div_recip(a.dz2), dr2, div_recip(dr2), o);
         if (l == 1 || l == a.L) {
             double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
             for (int k = 0; k < 8; ++k) rb[(int64_t)k * a.pitch + m] = o[k];
@@ -134,10 +134,20 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
 __device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, const double* raw, Rec& c) {
     for (int k = 0; k < 8; ++k) c.u[k] = raw[k];
     c.r = cell_r(a, l, m);
-    if (l == 0 || l == a.L + 1 || m > a.Mn) {
-        c.rho = c.vz = c.vr = c.vphi = c.p = c.P = c.Hz = c.Hr = c.Hphi = 0.0;
-    } else {
-        rec_phys(c, a.gm1);
+    rec_phys_shared(c, a.gm1);
+    // Selects, not a branch around rec_phys_shared: two definitions of the
+    // record merged at a join would put it in private memory.
+    const bool zero = (l == 0 || l == a.L + 1 || m > a.Mn);
+    if (__builtin_amdgcn_ballot_w64(zero) != 0) {
+        c.rho = zero ? 0.0 : c.rho;
+        c.vz = zero ? 0.0 : c.vz;
+        c.vr = zero ? 0.0 : c.vr;
+        c.vphi = zero ? 0.0 : c.vphi;
+        c.p = zero ? 0.0 : c.p;
+        c.P = zero ? 0.0 : c.P;
+        c.Hz = zero ? 0.0 : c.Hz;
+        c.Hr = zero ? 0.0 : c.Hr;
+        c.Hphi = zero ? 0.0 : c.Hphi;
     }
 }
 
@@ -194,19 +204,55 @@ struct StateIO {
     }
 };
 
-__device__ __forceinline__ void shfl_rec(const Rec& c, Rec& e, Rec& w) {
+// Lane exchange by DPP wave shifts, one v_mov_b32_dpp per 32-bit half.
+// DPP_WAVE_SHR1 gives lane i the value of lane i-1, DPP_WAVE_SHL1 the value
+// of lane i+1; the lane without a source (0, resp. 63) keeps `edge`.
+#define DPP_WAVE_SHL1 0x130
+#define DPP_WAVE_SHR1 0x138
+template <int kCtrl>
+__device__ __forceinline__ double dpp_shift(double v, double edge) {
+    const long long vi = __double_as_longlong(v), ei = __double_as_longlong(edge);
+    const int lo = __builtin_amdgcn_update_dpp((int)ei, (int)vi, kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(ei >> 32), (int)(vi >> 32), kCtrl, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The values lf_cell reads from an m +- 1 neighbour, as one LDS entry.
+enum { HX_U = 0, HX_VZ = 8, HX_VR, HX_VPHI, HX_P, HX_HZ, HX_HR, HX_HPHI, HX_R };
+#define HALO_STRIDE 17  // doubles per entry (odd, so the lanes' entries spread over the banks)
+#define HALO_MAX_ROWS 48
+
+__device__ __forceinline__ void halo_put(double* e, const Rec& c) {
+    for (int k = 0; k < 8; ++k) e[HX_U + k] = c.u[k];
+    e[HX_VZ] = c.vz;
+    e[HX_VR] = c.vr;
+    e[HX_VPHI] = c.vphi;
+    e[HX_P] = c.P;
+    e[HX_HZ] = c.Hz;
+    e[HX_HR] = c.Hr;
+    e[HX_HPHI] = c.Hphi;
+    e[HX_R] = c.r;
+}
+
+// E and W records of the row whose centre records are c: the adjacent
+// lanes, and for lanes 0 and 63 the tile's edge cells we / ee (LDS entries
+// every lane reads; the shifts use them only in the lane without a source).
+__device__ __forceinline__ void exchange(const Rec& c, const double* we, const double* ee, Rec& e, Rec& w) {
+#define MPD_XCH(field, slot)                                        \
+    w.field = dpp_shift<DPP_WAVE_SHR1>(c.field, we[slot]);          \
+    e.field = dpp_shift<DPP_WAVE_SHL1>(c.field, ee[slot]);
     for (int k = 0; k < 8; ++k) {
-        e.u[k] = __shfl_down(c.u[k], 1, 64);
-        w.u[k] = __shfl_up(c.u[k], 1, 64);
+        MPD_XCH(u[k], HX_U + k)
     }
-    e.vz = __shfl_down(c.vz, 1, 64);     w.vz = __shfl_up(c.vz, 1, 64);
-    e.vr = __shfl_down(c.vr, 1, 64);     w.vr = __shfl_up(c.vr, 1, 64);
-    e.vphi = __shfl_down(c.vphi, 1, 64); w.vphi = __shfl_up(c.vphi, 1, 64);
-    e.P = __shfl_down(c.P, 1, 64);       w.P = __shfl_up(c.P, 1, 64);
-    e.Hz = __shfl_down(c.Hz, 1, 64);     w.Hz = __shfl_up(c.Hz, 1, 64);
-    e.Hr = __shfl_down(c.Hr, 1, 64);     w.Hr = __shfl_up(c.Hr, 1, 64);
-    e.Hphi = __shfl_down(c.Hphi, 1, 64); w.Hphi = __shfl_up(c.Hphi, 1, 64);
-    e.r = __shfl_down(c.r, 1, 64);       w.r = __shfl_up(c.r, 1, 64);
+    MPD_XCH(vz, HX_VZ)
+    MPD_XCH(vr, HX_VR)
+    MPD_XCH(vphi, HX_VPHI)
+    MPD_XCH(P, HX_P)
+    MPD_XCH(Hz, HX_HZ)
+    MPD_XCH(Hr, HX_HR)
+    MPD_XCH(Hphi, HX_HPHI)
+    MPD_XCH(r, HX_R)
+#undef MPD_XCH
 }
 
 // Single-precision estimate of the wave speed v + c_s + c_a of conserved
@@ -243,11 +289,12 @@ __device__ __forceinline__ double speed_T(const StepArgs& a, int loop) {
 }
 
 struct TileCtx {
-    int m, l1;
+    int m, l0, l1, H;
     uint32_t moff;
     bool outlane;
     double dt, dz2, dz2i;
     float thr, gg1;
+    const double* halo;  // this wave's edge records: [west, east][H][HALO_STRIDE]
 };
 
 // One row of the march: output row l from the records S (l-1), C (l), N
@@ -260,7 +307,8 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
     double raw[8];
     if (pre) io.load_row(l + 2, raw);
     Rec E, W;
-    shfl_rec(C, E, W);
+    const double* we = x.halo + (l - x.l0) * HALO_STRIDE;
+    exchange(C, we, we + x.H * HALO_STRIDE, E, W);
     const double dr2 = 2 * a.dr[l];
     double o[8];
     lf_cell<Div>(N, S, E, W, C, x.dt, x.dz2, x.dz2i, dr2, a.dr2inv[l], o);
@@ -282,50 +330,81 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
     if (pre) row_rec(a, l + 2, x.m, raw, S);
 }
 
-// The production stencil.  Each wavefront owns a tile of 64 columns (62
-// outputs plus one halo column on each side) by H rows and marches down the
-// rows with the records of rows l-1, l, l+1 in registers (three slots used
-// round-robin, so nothing is copied between rows): every cell's conserved
-// values are read from HBM once and its physical values computed once; the
-// m +- 1 neighbours come from the adjacent lanes (ds_bpermute), the l +- 1
-// neighbours from the register window; each row's loads are issued a row
-// ahead of their use.  Tiles are ordered column-strip-fastest so that
-// neighbouring waves share halo columns in L2.
+// The production stencil.  Each wavefront owns a tile of 64 columns by H
+// rows, column 2 + 64 s + lane in lane `lane` of strip s, and marches down
+// the rows with the records of rows l-1, l, l+1 in registers (three slots
+// used round-robin, so nothing is copied between rows): every cell's
+// conserved values are read from HBM once and its physical values computed
+// once.  The m +- 1 neighbours come from the adjacent lanes by DPP shifts;
+// the two columns just outside the tile (west 1 + 64 s, east 66 + 64 s) are
+// read and turned into records once per tile, all H rows in one pass, and
+// kept in LDS.  With column 2 on a 128-byte boundary every row of 64 outputs
+// is four whole cache lines.  Each row's loads are issued a row ahead of
+// their use.
 template <bool kSpeed, bool kBuf, class Div>
-__global__ void __launch_bounds__(256) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
+__global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
                                                         int nstrips, int ntiles) {
+    extern __shared__ double halo_lds[];
     if (loop && a.st->done) return;
     const int lane = (int)(threadIdx.x & 63);
-    // Wave-uniform tile index (scalar registers: row addressing and the march
-    // loop then run on the scalar unit).
-    const int tile = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // Wave-uniform indices (scalar registers: row addressing and the march
+    // loop then run on the scalar unit).  Workgroups are dealt to the 8 XCDs
+    // round-robin (workgroup b on XCD b % 8); the launch uses a multiple of 8
+    // workgroups and gives each XCD a contiguous run of tiles, a band of rows,
+    // so shared halo rows and columns are read through one L2.
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int per = (int)(gridDim.x >> 3);
+    const int tile = ((int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3)) * 4 + wave;
     if (tile >= ntiles) return;
     const int strip = tile % nstrips, chunk = tile / nstrips;
     TileCtx x;
-    x.m = 1 + 62 * strip + lane;
-    const int l0 = 1 + chunk * H;
-    x.l1 = min(l0 + H, a.L + 1);
+    x.m = 2 + 64 * strip + lane;
+    x.l0 = 1 + chunk * H;
+    x.l1 = min(x.l0 + H, a.L + 1);
+    x.H = H;
     x.dt = step_dt(a, dt_explicit, loop);
     x.moff = (uint32_t)(min(x.m, a.Mn + 1)) * 8u;
-    x.outlane = lane >= 1 && lane <= 62 && x.m <= a.Mn - 1;
+    x.outlane = x.m <= a.Mn - 1;
     x.dz2 = a.dz2;
-    x.dz2i = 1.0 / a.dz2;
+    x.dz2i = div_recip(a.dz2);
     x.thr = kSpeed ? (float)speed_T(a, loop) * SPEED_CAND_FRAC : 0.0f;
     x.gg1 = (float)(a.gamma * a.gm1);
+    double* halo = halo_lds + (int64_t)wave * 2 * H * HALO_STRIDE;
+    x.halo = halo;
+    // Edge records, 32 rows per pass: lanes 0..31 the west column, lanes
+    // 32..63 the east column (the ghost column past the last strip).
+    for (int j0 = 0; j0 < H; j0 += 32) {
+        const int j = j0 + (lane & 31);
+        const int east = lane >> 5;
+        const int hl = min(x.l0 + j, x.l1 - 1);
+        const int hm = east ? min(66 + 64 * strip, a.Mn + 1) : 1 + 64 * strip;
+        const double* src = a.u_in + at(a, hl, hm);
+        double raw[8];
+        for (int k = 0; k < 8; ++k) raw[k] = src[(int64_t)k * a.fstride];
+        Rec h;
+        row_rec(a, hl, hm, raw, h);
+        if (j < H) halo_put(halo + (east * H + j) * HALO_STRIDE, h);
+    }
+    // The entries are read back by other lanes of this wave only: LDS
+    // operations of one wave complete in order; the fences keep the
+    // compiler from moving the reads above the writes.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     StateIO<kBuf> io;
     io.init(a, x.moff);
     Rec A, B, C;
     {
         double raw[8];
-        io.load_row(l0 - 1, raw);
-        row_rec(a, l0 - 1, x.m, raw, A);
-        io.load_row(l0, raw);
-        row_rec(a, l0, x.m, raw, B);
-        io.load_row(l0 + 1, raw);
-        row_rec(a, l0 + 1, x.m, raw, C);
+        io.load_row(x.l0 - 1, raw);
+        row_rec(a, x.l0 - 1, x.m, raw, A);
+        io.load_row(x.l0, raw);
+        row_rec(a, x.l0, x.m, raw, B);
+        io.load_row(x.l0 + 1, raw);
+        row_rec(a, x.l0 + 1, x.m, raw, C);
     }
     uint64_t key = 0;
-    int l = l0;
+    int l = x.l0;
     for (; l + 3 <= x.l1; l += 3) {
         march_row<kSpeed, kBuf, Div>(a, x, io, l, A, B, C, key);
         march_row<kSpeed, kBuf, Div>(a, x, io, l + 1, B, C, A, key);
@@ -923,6 +1002,14 @@ __global__ void __launch_bounds__(256) lf_radial_sym(StepArgs a, unsigned long l
     if ((threadIdx.x & 63) == 0 && key != 0) atomicMax(out, (unsigned long long)key);
 }
 
+// y[l] = div_recip(2 d[l]): the refined reciprocals of the per-row radial
+// divisors 2 dr_l (src/solver.cpp:97), made once per solver on the device
+// because they come from the device's v_rcp_f64.
+__global__ void lf_recip_table(const double* d, double* y, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) y[i] = div_recip(2 * d[i]);
+}
+
 // Fill kernel (used instead of memset nodes inside captured graphs).
 __global__ void lf_fill(double* p, int64_t n, double v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -947,30 +1034,33 @@ hipError_t launch_simple(const StepArgs& a, double dt, int loop, int speed, hipS
 
 template <class Div>
 hipError_t launch_tiled(const StepArgs& a, int H, double dt, int loop, int speed, hipStream_t s) {
-    const int nstrips = (a.Mn - 2 + 61) / 62;
+    const int nstrips = (a.Mn - 2 + 63) / 64;
     const int nchunks = (a.L + H - 1) / H;
     const int ntiles = nstrips * nchunks;
-    const unsigned blocks = (unsigned)((ntiles + 3) / 4);
+    // A multiple of 8 workgroups, so the kernel's XCD-aware tile order covers every tile.
+    const unsigned blocks = (unsigned)(((ntiles + 3) / 4 + 7) / 8 * 8);
+    const size_t lds = (size_t)4 * 2 * H * HALO_STRIDE * sizeof(double);
     // Buffer addressing needs every row/field offset of a state below 2^31 bytes.
     const bool buf = (double)a.fstride * 8.0 * 8.0 < 2147483648.0;
     if (buf) {
-        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
-        else hipLaunchKernelGGL((lf_stencil_tiled<false, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, true, Div>), dim3(blocks), dim3(256), lds, s, a, dt, loop, H, nstrips, ntiles);
+        else hipLaunchKernelGGL((lf_stencil_tiled<false, true, Div>), dim3(blocks), dim3(256), lds, s, a, dt, loop, H, nstrips, ntiles);
     } else {
-        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
-        else hipLaunchKernelGGL((lf_stencil_tiled<false, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, false, Div>), dim3(blocks), dim3(256), lds, s, a, dt, loop, H, nstrips, ntiles);
+        else hipLaunchKernelGGL((lf_stencil_tiled<false, false, Div>), dim3(blocks), dim3(256), lds, s, a, dt, loop, H, nstrips, ntiles);
     }
     return hipGetLastError();
 }
 
 int auto_rows(const StepArgs& a) {
-    // Enough tiles to give every CU several waves, rows long enough that the
-    // two halo rows per tile stay a small fraction of the work.
-    const int nstrips = (a.Mn - 2 + 61) / 62;
-    const long long target_tiles = 256LL * 4 * 6;
+    // Many short tiles: about 32 per SIMD, so the last wave of tiles leaves
+    // little of the GPU idle (measured at 6400x2560: H = 8 beats 16 and 32
+    // although each tile reads two extra rows, which mostly hit L2).
+    const int nstrips = (a.Mn - 2 + 63) / 64;
+    const long long target_tiles = 256LL * 4 * 32;
     long long H = ((long long)a.L * nstrips + target_tiles - 1) / target_tiles;
-    if (H < 16) H = 16;
-    if (H > 512) H = 512;
+    if (H < 8) H = 8;
+    if (H > 32) H = 32;
     return (int)H;
 }
 }  // namespace
@@ -981,7 +1071,8 @@ hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt, int
         if (c.fast_div && !a.step1) return launch_simple<DivConst>(a, dt, loop, speed, s);
         return launch_simple<DivExact>(a, dt, loop, speed, s);
     }
-    const int H = c.tile_rows > 0 ? c.tile_rows : auto_rows(a);
+    int H = c.tile_rows > 0 ? c.tile_rows : auto_rows(a);
+    if (H > HALO_MAX_ROWS) H = HALO_MAX_ROWS;
     if (c.fast_div) return launch_tiled<DivConst>(a, H, dt, loop, speed, s);
     return launch_tiled<DivExact>(a, H, dt, loop, speed, s);
 }
@@ -1032,6 +1123,11 @@ hipError_t launch_outlet(const StepArgs& a, int M_max, double* out, hipStream_t 
 
 hipError_t launch_radial_sym(const StepArgs& a, unsigned long long* out, hipStream_t s) {
     hipLaunchKernelGGL(lf_radial_sym, dim3(1024), dim3(256), 0, s, a, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_recip_table(const double* d, double* y, int n, hipStream_t s) {
+    hipLaunchKernelGGL(lf_recip_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, y, n);
     return hipGetLastError();
 }
 

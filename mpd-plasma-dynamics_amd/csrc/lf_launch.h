@@ -9,6 +9,12 @@ namespace mpd {
 
 enum StencilKind { STENCIL_SIMPLE = 0, STENCIL_TILED = 1 };
 
+// Offset, in doubles, of column 0 from the start of every pitched field
+// buffer: with a row pitch that is a multiple of 16 doubles and a 256-byte
+// aligned allocation, column 2 (the first central column) and so every
+// 64-column strip of the tiled stencil starts a 128-byte cache line.
+constexpr int kColOff = 14;
+
 struct StencilCfg {
     int kind = STENCIL_TILED;
     bool fast_div = true;
@@ -34,5 +40,7 @@ hipError_t launch_integrals(const StepArgs& a, double* rowsum, double* out, hipS
 hipError_t launch_outlet(const StepArgs& a, int M_max, double* out, hipStream_t s);
 hipError_t launch_radial_sym(const StepArgs& a, unsigned long long* out, hipStream_t s);
 hipError_t launch_fill(double* p, long long n, double v, hipStream_t s);
+// y[i] = div_recip(2 d[i]) for i < n.
+hipError_t launch_recip_table(const double* d, double* y, int n, hipStream_t s);
 
 }  // namespace mpd

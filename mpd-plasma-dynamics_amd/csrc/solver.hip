@@ -208,7 +208,10 @@ struct mpd_solver {
     ~mpd_solver() {
         for (auto& g : graph)
             if (g) (void)hipGraphExecDestroy(g);
-        double* bufs[] = {state[0], state[1], rowbuf, phys_ic, prev, partial, rowsum, scratch, geo};
+        double* pitched[] = {state[0], state[1], rowbuf, phys_ic, prev};
+        for (double* b : pitched)
+            if (b) (void)hipFree(RawBase(b));
+        double* bufs[] = {partial, rowsum, scratch, geo};
         for (double* b : bufs)
             if (b) (void)hipFree(b);
         if (bc) (void)hipFree(bc);
@@ -260,6 +263,15 @@ struct mpd_solver {
         return static_cast<double*>(p);
     }
 
+    // Pitched field buffers: nrow rows of `pitch` doubles, with element (row,
+    // m) at p[row * pitch + m] for the returned p = allocation + kColOff, so
+    // that column 2 (the first central column) starts a 128-byte line.
+    double* AllocPitched(size_t nrow) { return Alloc(nrow * (size_t)pitch) + mpd::kColOff; }
+    static double* RawBase(double* p) { return p ? p - mpd::kColOff : nullptr; }
+    void ZeroPitched(double* p, size_t nrow) {
+        HIPCHK(hipMemset(RawBase(p), 0, nrow * (size_t)pitch * sizeof(double)));
+    }
+
     // Copy nf host arrays (rows x cols) into a pitched device buffer with the
     // ghost ring zeroed.
     void UploadFields(double* dst, const double* const* src, int nf) {
@@ -267,8 +279,9 @@ struct mpd_solver {
         for (int k = 0; k < nf; ++k) {
             std::fill(stage.begin(), stage.end(), 0.0);
             for (int l = 1; l <= grid.L; ++l)
-                std::memcpy(&stage[(size_t)l * pitch + 1], src[k] + (size_t)l * cols + 1, sizeof(double) * grid.Mn);
-            HIPCHK(hipMemcpy(dst + (int64_t)k * fstride, stage.data(), stage.size() * sizeof(double),
+                std::memcpy(&stage[(size_t)l * pitch + mpd::kColOff + 1], src[k] + (size_t)l * cols + 1,
+                            sizeof(double) * grid.Mn);
+            HIPCHK(hipMemcpy(RawBase(dst) + (int64_t)k * fstride, stage.data(), stage.size() * sizeof(double),
                              hipMemcpyHostToDevice));
         }
     }
@@ -289,20 +302,19 @@ struct mpd_solver {
         steps_per_graph = cfg.gpu.steps_per_graph;
         rows = grid.rows;
         cols = grid.cols;
-        pitch = (cols + 15) / 16 * 16;
+        pitch = (cols + mpd::kColOff + 15) / 16 * 16;
         fstride = (int64_t)rows * pitch;
         mpd::HostFields& f = h.f;
         const MpdBcProgram& hbc = h.bc;
 
         HIPCHK(hipGetDevice(&device));
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        const size_t n8 = (size_t)8 * fstride;
-        state[0] = Alloc(n8);
-        state[1] = Alloc(n8);
-        HIPCHK(hipMemset(state[0], 0, n8 * sizeof(double)));
-        HIPCHK(hipMemset(state[1], 0, n8 * sizeof(double)));
-        rowbuf = Alloc((size_t)16 * pitch);
-        HIPCHK(hipMemset(rowbuf, 0, (size_t)16 * pitch * sizeof(double)));
+        state[0] = AllocPitched((size_t)8 * rows);
+        state[1] = AllocPitched((size_t)8 * rows);
+        ZeroPitched(state[0], (size_t)8 * rows);
+        ZeroPitched(state[1], (size_t)8 * rows);
+        rowbuf = AllocPitched(16);
+        ZeroPitched(rowbuf, 16);
         partial = Alloc(2 * 1024);
         rowsum = Alloc((size_t)7 * rows);
         scratch = Alloc(64);
@@ -314,9 +326,11 @@ struct mpd_solver {
             g[2 * rows + l] = grid.drin[l];
             g[3 * rows + l] = grid.drout[l];
             g[4 * rows + l] = grid.dr[l];
-            g[5 * rows + l] = 1.0 / (2 * grid.dr[l]);
+            g[5 * rows + l] = 0.0;  // div_recip(2 dr), filled on the device below
         }
         HIPCHK(hipMemcpy(geo, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(mpd::launch_recip_table(geo + 4 * rows, geo + 5 * rows, rows, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipMalloc((void**)&bc, sizeof(MpdBcProgram)));
         HIPCHK(hipMemcpy(bc, &hbc, sizeof(MpdBcProgram), hipMemcpyHostToDevice));
         HIPCHK(hipMalloc((void**)&st, sizeof(MpdLoopState)));
@@ -336,7 +350,7 @@ struct mpd_solver {
     }
 
     void EnsurePrevFromPhys(const double* const* php) {
-        if (!prev) prev = Alloc((size_t)7 * fstride);
+        if (!prev) prev = AllocPitched((size_t)7 * rows);
         const double* p7[7] = {php[mpd::F_RHO], php[mpd::F_VZ], php[mpd::F_VR], php[mpd::F_VPHI],
                                php[mpd::F_HZ], php[mpd::F_HR], php[mpd::F_HPHI]};
         UploadFields(prev, p7, 7);
@@ -345,9 +359,9 @@ struct mpd_solver {
     void UploadState(const double* const* u0, const double* const* phys) {
         HIPCHK(hipStreamSynchronize(stream));
         UploadFields(state[cur], u0, 8);
-        HIPCHK(hipMemset(state[1 - cur], 0, (size_t)8 * fstride * sizeof(double)));
+        ZeroPitched(state[1 - cur], (size_t)8 * rows);
         if (phys) {
-            if (!phys_ic) phys_ic = Alloc((size_t)10 * fstride);
+            if (!phys_ic) phys_ic = AllocPitched((size_t)10 * rows);
             UploadFields(phys_ic, phys, 10);
             step1 = true;
         } else {
@@ -396,8 +410,8 @@ struct mpd_solver {
     double SolutionChange(bool save) {
         if (!prev) {
             // No snapshot yet: take the current state as the snapshot first.
-            prev = Alloc((size_t)7 * fstride);
-            HIPCHK(hipMemsetAsync(prev, 0, (size_t)7 * fstride * sizeof(double), stream));
+            prev = AllocPitched((size_t)7 * rows);
+            HIPCHK(hipMemsetAsync(RawBase(prev), 0, (size_t)7 * fstride * sizeof(double), stream));
             HIPCHK(mpd::launch_residual(Args(cur), prev, partial, 0, 1, 0.0, 1, stream));
         }
         HIPCHK(mpd::launch_residual(Args(cur), prev, partial, 0, 1, 0.0, save ? 1 : 0, stream));
@@ -669,18 +683,18 @@ int mpd_download(mpd_solver* s, double* const* u0, double* const* phys) {
             if (s->step1) {
                 s->DownloadFields(s->phys_ic, phys, 10);
             } else {
-                double* tmp = s->Alloc((size_t)10 * s->fstride);
+                double* tmp = s->AllocPitched((size_t)10 * s->rows);
                 const hipError_t e = mpd::launch_phys_fields(s->Args(s->cur), tmp, s->stream);
                 if (e == hipSuccess) {
                     (void)hipStreamSynchronize(s->stream);
                     try {
                         s->DownloadFields(tmp, phys, 10);
                     } catch (...) {
-                        (void)hipFree(tmp);
+                        (void)hipFree(mpd_solver::RawBase(tmp));
                         throw;
                     }
                 }
-                (void)hipFree(tmp);
+                (void)hipFree(mpd_solver::RawBase(tmp));
                 HIPCHK(e);
             }
         }
