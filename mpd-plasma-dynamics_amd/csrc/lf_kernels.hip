@@ -158,6 +158,10 @@ __device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, const d
 // buffer must stay below 2^31 bytes).  Without kBuf, plain pointers.
 // Lanes past the ghost column address the ghost column, which holds 0 in
 // every state buffer (never written).
+// Cache policy of the state stores: non-temporal (the new state is not read
+// again in this launch; measured 5% faster than the default policy).
+constexpr int kStoreNT = 2;
+
 template <bool kBuf>
 struct StateIO {
     __amdgpu_buffer_rsrc_t rin, rout;
@@ -195,7 +199,7 @@ struct StateIO {
             using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(rin, 0, 0, 0));
             for (int k = 0; k < 8; ++k)
                 __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, o[k]), rout, moff,
-                                                      so + (int)(k * field_bytes), 0);
+                                                      so + (int)(k * field_bytes), kStoreNT);
         } else {
             char* row = reinterpret_cast<char*>(pout) + l * row_bytes + moff;
             for (int k = 0; k < 8; ++k)
@@ -348,13 +352,12 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt
     if (loop && a.st->done) return;
     const int lane = (int)(threadIdx.x & 63);
     // Wave-uniform indices (scalar registers: row addressing and the march
-    // loop then run on the scalar unit).  Workgroups are dealt to the 8 XCDs
-    // round-robin (workgroup b on XCD b % 8); the launch uses a multiple of 8
-    // workgroups and gives each XCD a contiguous run of tiles, a band of rows,
-    // so shared halo rows and columns are read through one L2.
+    // loop then run on the scalar unit).  Tiles in launch order, strip
+    // fastest: the four waves of a workgroup take four adjacent strips of one
+    // band of rows, and the whole GPU works on a narrow band at a time (an
+    // order giving each XCD its own band of rows measured 5% slower).
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int per = (int)(gridDim.x >> 3);
-    const int tile = ((int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3)) * 4 + wave;
+    const int tile = (int)blockIdx.x * 4 + wave;
     if (tile >= ntiles) return;
     const int strip = tile % nstrips, chunk = tile / nstrips;
     TileCtx x;
@@ -1037,8 +1040,7 @@ hipError_t launch_tiled(const StepArgs& a, int H, double dt, int loop, int speed
     const int nstrips = (a.Mn - 2 + 63) / 64;
     const int nchunks = (a.L + H - 1) / H;
     const int ntiles = nstrips * nchunks;
-    // A multiple of 8 workgroups, so the kernel's XCD-aware tile order covers every tile.
-    const unsigned blocks = (unsigned)(((ntiles + 3) / 4 + 7) / 8 * 8);
+    const unsigned blocks = (unsigned)((ntiles + 3) / 4);
     const size_t lds = (size_t)4 * 2 * H * HALO_STRIDE * sizeof(double);
     // Buffer addressing needs every row/field offset of a state below 2^31 bytes.
     const bool buf = (double)a.fstride * 8.0 * 8.0 < 2147483648.0;
