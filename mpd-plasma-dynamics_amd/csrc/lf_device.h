@@ -218,6 +218,92 @@ __device__ __forceinline__ void lf_cell(const Rec& N, const Rec& S, const Rec& E
                  D(N.u[7] * N.vz - S.u[7] * S.vz, dz2, dz2i));
 }
 
+// Radial record of a cell: what lf_cell reads from an m +- 1 neighbour,
+// the radial flux products (each in the reference's operation order) and
+// then the 8 conserved values (entry XB_U + k).
+enum {
+    XB_U0VR = 0, XB_HZHRR, XB_U1VR, XB_HRHRPR, XB_U2VR, XB_HPHIHRR, XB_U3VR, XB_VRR,
+    XB_U4VR, XB_HRVPHI, XB_U5VR, XB_HRVZR, XB_U6VR, XB_U, XB_N = XB_U + 8
+};
+
+// Solver::ComputeCentralUpdate's cell update (src/solver.cpp:94-150) with the
+// m +- 1 neighbours given as radial records: nb.E(i) / nb.W(i) is entry i of
+// the E / W neighbour's radial record.  Same operations in the same order as
+// lf_cell.  The scheduling barriers between the eight fields keep the
+// neighbour values of one field from being fetched while another is computed
+// (bounded register live ranges: the tiled stencil then needs no spills).
+template <class Div, class NB>
+__device__ __forceinline__ void lf_cell_rad(const Rec& N, const Rec& S, const Rec& C, const NB& nb,
+                                            double dt, double dz2, double dz2i, double dr2, double dr2i,
+                                            double* o) {
+    const Div D;
+#define MPD_FIELD_FENCE __builtin_amdgcn_sched_barrier(0)
+    o[0] = 0.25 * (N.u[0] + S.u[0] + nb.E(XB_U + 0) + nb.W(XB_U + 0)) +
+           dt * (D(-(N.u[0] * N.vz - S.u[0] * S.vz), dz2, dz2i) -
+                 D(nb.E(XB_U0VR) - nb.W(XB_U0VR), dr2, dr2i));
+    MPD_FIELD_FENCE;
+    o[1] = 0.25 * (N.u[1] + S.u[1] + nb.E(XB_U + 1) + nb.W(XB_U + 1)) +
+           dt * (D((N.Hz * N.Hz - N.P) * N.r - (S.Hz * S.Hz - S.P) * S.r, dz2, dz2i) +
+                 D(nb.E(XB_HZHRR) - nb.W(XB_HZHRR), dr2, dr2i) -
+                 D(N.u[1] * N.vz - S.u[1] * S.vz, dz2, dz2i) -
+                 D(nb.E(XB_U1VR) - nb.W(XB_U1VR), dr2, dr2i));
+    MPD_FIELD_FENCE;
+    o[2] = 0.25 * (N.u[2] + S.u[2] + nb.E(XB_U + 2) + nb.W(XB_U + 2)) +
+           dt * ((C.rho * C.vphi * C.vphi + C.P - C.Hphi * C.Hphi) +
+                 D(N.Hz * N.Hr * N.r - S.Hz * S.Hr * S.r, dz2, dz2i) +
+                 D(nb.E(XB_HRHRPR) - nb.W(XB_HRHRPR), dr2, dr2i) -
+                 D(N.u[2] * N.vz - S.u[2] * S.vz, dz2, dz2i) -
+                 D(nb.E(XB_U2VR) - nb.W(XB_U2VR), dr2, dr2i));
+    MPD_FIELD_FENCE;
+    o[3] = 0.25 * (N.u[3] + S.u[3] + nb.E(XB_U + 3) + nb.W(XB_U + 3)) +
+           dt * ((-C.rho * C.vr * C.vphi + C.Hphi * C.Hr) +
+                 D(N.Hphi * N.Hz * N.r - S.Hphi * S.Hz * S.r, dz2, dz2i) +
+                 D(nb.E(XB_HPHIHRR) - nb.W(XB_HPHIHRR), dr2, dr2i) -
+                 D(N.u[3] * N.vz - S.u[3] * S.vz, dz2, dz2i) -
+                 D(nb.E(XB_U3VR) - nb.W(XB_U3VR), dr2, dr2i));
+    MPD_FIELD_FENCE;
+    o[4] = 0.25 * (N.u[4] + S.u[4] + nb.E(XB_U + 4) + nb.W(XB_U + 4)) +
+           dt * (-C.p * (D(N.vz * N.r - S.vz * S.r, dz2, dz2i) +
+                         D(nb.E(XB_VRR) - nb.W(XB_VRR), dr2, dr2i)) -
+                 D(N.u[4] * N.vz - S.u[4] * S.vz, dz2, dz2i) -
+                 D(nb.E(XB_U4VR) - nb.W(XB_U4VR), dr2, dr2i));
+    MPD_FIELD_FENCE;
+    o[5] = 0.25 * (N.u[5] + S.u[5] + nb.E(XB_U + 5) + nb.W(XB_U + 5)) +
+           dt * (D(N.Hz * N.vphi - S.Hz * S.vphi, dz2, dz2i) +
+                 D(nb.E(XB_HRVPHI) - nb.W(XB_HRVPHI), dr2, dr2i) -
+                 D(N.u[5] * N.vz - S.u[5] * S.vz, dz2, dz2i) -
+                 D(nb.E(XB_U5VR) - nb.W(XB_U5VR), dr2, dr2i));
+    MPD_FIELD_FENCE;
+    o[6] = 0.25 * (N.u[6] + S.u[6] + nb.E(XB_U + 6) + nb.W(XB_U + 6)) +
+           dt * (D(nb.E(XB_HRVZR) - nb.W(XB_HRVZR), dr2, dr2i) -
+                 D(nb.E(XB_U6VR) - nb.W(XB_U6VR), dr2, dr2i));
+    MPD_FIELD_FENCE;
+    o[7] = 0.25 * (N.u[7] + S.u[7] + nb.E(XB_U + 7) + nb.W(XB_U + 7)) +
+           dt * (D(N.Hz * N.vr * N.r - S.Hz * S.vr * S.r, dz2, dz2i) -
+                 D(N.u[7] * N.vz - S.u[7] * S.vz, dz2, dz2i));
+#undef MPD_FIELD_FENCE
+}
+
+// Entry i of the radial record of cell c.
+__device__ __forceinline__ double radial_entry(const Rec& c, int i) {
+    switch (i) {
+        case XB_U0VR: return c.u[0] * c.vr;
+        case XB_HZHRR: return c.Hz * c.Hr * c.r;
+        case XB_U1VR: return c.u[1] * c.vr;
+        case XB_HRHRPR: return (c.Hr * c.Hr - c.P) * c.r;
+        case XB_U2VR: return c.u[2] * c.vr;
+        case XB_HPHIHRR: return c.Hphi * c.Hr * c.r;
+        case XB_U3VR: return c.u[3] * c.vr;
+        case XB_VRR: return c.vr * c.r;
+        case XB_U4VR: return c.u[4] * c.vr;
+        case XB_HRVPHI: return c.Hr * c.vphi;
+        case XB_U5VR: return c.u[5] * c.vr;
+        case XB_HRVZR: return c.Hr * c.vz * c.r;
+        case XB_U6VR: return c.u[6] * c.vr;
+        default: return c.u[i - XB_U];
+    }
+}
+
 // Max of non-negative doubles through their bit patterns; NaN is skipped as
 // std::max(local_max, v) skips it in src/diagnostics.cpp:36.
 __device__ __forceinline__ uint64_t speed_key(double s) {

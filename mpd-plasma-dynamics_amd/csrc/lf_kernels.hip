@@ -28,7 +28,7 @@ __device__ __forceinline__ const double* fld(const double* base, int64_t fstride
 }
 
 __device__ __forceinline__ int64_t at(const StepArgs& a, int l, int m) {
-    return (int64_t)l * a.pitch + m;
+    return (int64_t)l * a.pitch + phys_col(m);
 }
 
 // Start-of-step physical value set at an interior cell: f(u0), or the IC
@@ -51,7 +51,7 @@ __device__ __forceinline__ void s0_phys(const StepArgs& a, int l, int m, double*
         double u[8];
         if (l == 1 || l == a.L) {
             const double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
-            for (int k = 0; k < 8; ++k) u[k] = rb[(int64_t)k * a.pitch + m];
+            for (int k = 0; k < 8; ++k) u[k] = rb[(int64_t)k * a.pitch + phys_col(m)];
         } else {
             for (int k = 0; k < 8; ++k) u[k] = fld(a.u_out, a.fstride, k)[at(a, l, m)];
         }
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
         lf_cell<Div>(N, S, E, W, C, dt, a.dz2, div_recip(a.dz2), dr2, div_recip(dr2), o);
         if (l == 1 || l == a.L) {
             double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
-            for (int k = 0; k < 8; ++k) rb[(int64_t)k * a.pitch + m] = o[k];
+            for (int k = 0; k < 8; ++k) rb[(int64_t)k * a.pitch + phys_col(m)] = o[k];
         } else {
             const int64_t i = at(a, l, m);
             for (int k = 0; k < 8; ++k) a.u_out[(int64_t)k * a.fstride + i] = o[k];
@@ -137,8 +137,8 @@ __device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, const d
     rec_phys_shared(c, a.gm1);
     // Selects, not a branch around rec_phys_shared: two definitions of the
     // record merged at a join would put it in private memory.
-    const bool zero = (l == 0 || l == a.L + 1 || m > a.Mn);
-    if (__builtin_amdgcn_ballot_w64(zero) != 0) {
+    const bool zero = (l == 0 || l == a.L + 1 || m < 1 || m > a.Mn);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(zero) != 0, 0)) {
         c.rho = zero ? 0.0 : c.rho;
         c.vz = zero ? 0.0 : c.vz;
         c.vr = zero ? 0.0 : c.vr;
@@ -155,9 +155,8 @@ __device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, const d
 // instructions: a scalar descriptor for the buffer, the row and field offset
 // in a scalar register and this lane's column offset in a 32-bit vector
 // register, so no per-lane 64-bit address arithmetic (the whole 8-field
-// buffer must stay below 2^31 bytes).  Without kBuf, plain pointers.
-// Lanes past the ghost column address the ghost column, which holds 0 in
-// every state buffer (never written).
+// buffer must stay below 2^31 bytes).  Without kBuf, plain pointers.  Loads
+// and stores have their own lane offsets (see lf_stencil_tiled).
 // Cache policy of the state stores: non-temporal (the new state is not read
 // again in this launch; measured 5% faster than the default policy).
 constexpr int kStoreNT = 2;
@@ -168,10 +167,11 @@ struct StateIO {
     const double* pin;
     double* pout;
     int64_t row_bytes, field_bytes;
-    uint32_t moff;
+    uint32_t ld_off, st_off;
 
-    __device__ __forceinline__ void init(const StepArgs& a, uint32_t moff_) {
-        moff = moff_;
+    __device__ __forceinline__ void init(const StepArgs& a, uint32_t ld, uint32_t st) {
+        ld_off = ld;
+        st_off = st;
         row_bytes = (int64_t)a.pitch * 8;
         field_bytes = a.fstride * 8;
         pin = a.u_in;
@@ -185,11 +185,11 @@ struct StateIO {
         if (kBuf) {
             const int so = (int)(l * row_bytes);
             for (int k = 0; k < 8; ++k) {
-                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rin, moff, so + (int)(k * field_bytes), 0);
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rin, ld_off, so + (int)(k * field_bytes), 0);
                 raw[k] = __builtin_bit_cast(double, v);
             }
         } else {
-            const char* row = reinterpret_cast<const char*>(pin) + l * row_bytes + moff;
+            const char* row = reinterpret_cast<const char*>(pin) + l * row_bytes + ld_off;
             for (int k = 0; k < 8; ++k) raw[k] = *reinterpret_cast<const double*>(row + k * field_bytes);
         }
     }
@@ -198,66 +198,37 @@ struct StateIO {
             const int so = (int)(l * row_bytes);
             using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(rin, 0, 0, 0));
             for (int k = 0; k < 8; ++k)
-                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, o[k]), rout, moff,
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, o[k]), rout, st_off,
                                                       so + (int)(k * field_bytes), kStoreNT);
         } else {
-            char* row = reinterpret_cast<char*>(pout) + l * row_bytes + moff;
+            char* row = reinterpret_cast<char*>(pout) + l * row_bytes + st_off;
             for (int k = 0; k < 8; ++k)
                 __builtin_nontemporal_store(o[k], reinterpret_cast<double*>(row + k * field_bytes));
         }
     }
 };
 
-// Lane exchange by DPP wave shifts, one v_mov_b32_dpp per 32-bit half.
+// Lane exchange by DPP wave shifts, one v_mov_b32_dpp per 32-bit half:
 // DPP_WAVE_SHR1 gives lane i the value of lane i-1, DPP_WAVE_SHL1 the value
-// of lane i+1; the lane without a source (0, resp. 63) keeps `edge`.
+// of lane i+1 (the lane without a source gets 0; it is an edge lane whose
+// result is not used).
 #define DPP_WAVE_SHL1 0x130
 #define DPP_WAVE_SHR1 0x138
 template <int kCtrl>
-__device__ __forceinline__ double dpp_shift(double v, double edge) {
-    const long long vi = __double_as_longlong(v), ei = __double_as_longlong(edge);
-    const int lo = __builtin_amdgcn_update_dpp((int)ei, (int)vi, kCtrl, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(ei >> 32), (int)(vi >> 32), kCtrl, 0xf, 0xf, false);
+__device__ __forceinline__ double dpp_shift(double v) {
+    const long long vi = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)vi, kCtrl, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(vi >> 32), kCtrl, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// The values lf_cell reads from an m +- 1 neighbour, as one LDS entry.
-enum { HX_U = 0, HX_VZ = 8, HX_VR, HX_VPHI, HX_P, HX_HZ, HX_HR, HX_HPHI, HX_R };
-#define HALO_STRIDE 17  // doubles per entry (odd, so the lanes' entries spread over the banks)
-#define HALO_MAX_ROWS 48
-
-__device__ __forceinline__ void halo_put(double* e, const Rec& c) {
-    for (int k = 0; k < 8; ++k) e[HX_U + k] = c.u[k];
-    e[HX_VZ] = c.vz;
-    e[HX_VR] = c.vr;
-    e[HX_VPHI] = c.vphi;
-    e[HX_P] = c.P;
-    e[HX_HZ] = c.Hz;
-    e[HX_HR] = c.Hr;
-    e[HX_HPHI] = c.Hphi;
-    e[HX_R] = c.r;
-}
-
-// E and W records of the row whose centre records are c: the adjacent
-// lanes, and for lanes 0 and 63 the tile's edge cells we / ee (LDS entries
-// every lane reads; the shifts use them only in the lane without a source).
-__device__ __forceinline__ void exchange(const Rec& c, const double* we, const double* ee, Rec& e, Rec& w) {
-#define MPD_XCH(field, slot)                                        \
-    w.field = dpp_shift<DPP_WAVE_SHR1>(c.field, we[slot]);          \
-    e.field = dpp_shift<DPP_WAVE_SHL1>(c.field, ee[slot]);
-    for (int k = 0; k < 8; ++k) {
-        MPD_XCH(u[k], HX_U + k)
-    }
-    MPD_XCH(vz, HX_VZ)
-    MPD_XCH(vr, HX_VR)
-    MPD_XCH(vphi, HX_VPHI)
-    MPD_XCH(P, HX_P)
-    MPD_XCH(Hz, HX_HZ)
-    MPD_XCH(Hr, HX_HR)
-    MPD_XCH(Hphi, HX_HPHI)
-    MPD_XCH(r, HX_R)
-#undef MPD_XCH
-}
+// Neighbour access of lf_cell_rad: each lane computes the radial record
+// entry of its own cell and the DPP shift brings the E / W neighbour's.
+struct LaneNb {
+    const Rec& c;
+    __device__ __forceinline__ double E(int i) const { return dpp_shift<DPP_WAVE_SHL1>(radial_entry(c, i)); }
+    __device__ __forceinline__ double W(int i) const { return dpp_shift<DPP_WAVE_SHR1>(radial_entry(c, i)); }
+};
 
 // Single-precision estimate of the wave speed v + c_s + c_a of conserved
 // values u at radius r (gg1 = gamma (gamma - 1), so c_s^2 = gg1 e).  Its
@@ -293,12 +264,11 @@ __device__ __forceinline__ double speed_T(const StepArgs& a, int loop) {
 }
 
 struct TileCtx {
-    int m, l0, l1, H;
-    uint32_t moff;
-    bool outlane;
+    int m, l1;
+    uint32_t st_off;
+    bool outlane, store;
     double dt, dz2, dz2i;
     float thr, gg1;
-    const double* halo;  // this wave's edge records: [west, east][H][HALO_STRIDE]
 };
 
 // One row of the march: output row l from the records S (l-1), C (l), N
@@ -310,45 +280,44 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
     const bool pre = (l + 2 <= x.l1);
     double raw[8];
     if (pre) io.load_row(l + 2, raw);
-    Rec E, W;
-    const double* we = x.halo + (l - x.l0) * HALO_STRIDE;
-    exchange(C, we, we + x.H * HALO_STRIDE, E, W);
+    const LaneNb nb{C};
     const double dr2 = 2 * a.dr[l];
     double o[8];
-    lf_cell<Div>(N, S, E, W, C, x.dt, x.dz2, x.dz2i, dr2, a.dr2inv[l], o);
-    if (x.outlane) {
-        if (l == 1 || l == a.L) {
-            char* rb = reinterpret_cast<char*>(a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch) + x.moff;
+    lf_cell_rad<Div>(N, S, C, nb, x.dt, x.dz2, x.dz2i, dr2, a.dr2inv[l], o);
+    if (l == 1 || l == a.L) {
+        if (x.outlane) {
+            char* rb = reinterpret_cast<char*>(a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch) + x.st_off;
             for (int k = 0; k < 8; ++k) *reinterpret_cast<double*>(rb + (int64_t)k * a.pitch * 8) = o[k];
-        } else {
-            io.store_row(l, o);
-            if (kSpeed) {
-                const float est = speed_estimate(o, C.r, x.gg1);
-                if (!(est < x.thr)) {
-                    const uint64_t kk = speed_key(wave_speed_u(o, C.r, a.gamma, a.gm1));
-                    key = (kk > key) ? kk : key;
-                }
+        }
+    } else {
+        if (x.store) io.store_row(l, o);
+        if (kSpeed && x.outlane) {
+            const float est = speed_estimate(o, C.r, x.gg1);
+            if (!(est < x.thr)) {
+                const uint64_t kk = speed_key(wave_speed_u(o, C.r, a.gamma, a.gm1));
+                key = (kk > key) ? kk : key;
             }
         }
     }
     if (pre) row_rec(a, l + 2, x.m, raw, S);
 }
 
-// The production stencil.  Each wavefront owns a tile of 64 columns by H
-// rows, column 2 + 64 s + lane in lane `lane` of strip s, and marches down
-// the rows with the records of rows l-1, l, l+1 in registers (three slots
-// used round-robin, so nothing is copied between rows): every cell's
-// conserved values are read from HBM once and its physical values computed
-// once.  The m +- 1 neighbours come from the adjacent lanes by DPP shifts;
-// the two columns just outside the tile (west 1 + 64 s, east 66 + 64 s) are
-// read and turned into records once per tile, all H rows in one pass, and
-// kept in LDS.  With column 2 on a 128-byte boundary every row of 64 outputs
-// is four whole cache lines.  Each row's loads are issued a row ahead of
-// their use.
+// The production stencil.  Each wavefront owns strip s (columns 62 s ..
+// 62 s + 61, one 512-byte slot per field row) for H rows: lane i holds
+// column 62 s - 1 + i, so lanes 1..62 are the strip's cells and lanes 0 and
+// 63 the neighbouring strips' edge cells, read in the same row load.  The
+// wave marches down the rows with the records of rows l-1, l, l+1 in
+// registers (three slots used round-robin, so nothing is copied between
+// rows): every cell's conserved values are read from HBM once per tile and
+// its physical values computed once; the m +- 1 neighbours come from the
+// adjacent lanes by DPP shifts, the l +- 1 neighbours from the register
+// window; each row's loads are issued a row ahead of their use.  The store of
+// a row covers the whole slot: lane i writes slot (i + 63) mod 64, its own
+// cell for lanes 1..62 and an unused double for lanes 0 and 63; lanes of the
+// boundary and ghost columns 0, 1, M_max+1, M_max+2 do not store.
 template <bool kSpeed, bool kBuf, class Div>
 __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
-                                                        int nstrips, int ntiles) {
-    extern __shared__ double halo_lds[];
+                                                           int nstrips, int ntiles) {
     if (loop && a.st->done) return;
     const int lane = (int)(threadIdx.x & 63);
     // Wave-uniform indices (scalar registers: row addressing and the march
@@ -361,53 +330,33 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt
     if (tile >= ntiles) return;
     const int strip = tile % nstrips, chunk = tile / nstrips;
     TileCtx x;
-    x.m = 2 + 64 * strip + lane;
-    x.l0 = 1 + chunk * H;
-    x.l1 = min(x.l0 + H, a.L + 1);
-    x.H = H;
+    x.m = kStripCols * strip - 1 + lane;
+    const int l0 = 1 + chunk * H;
+    x.l1 = min(l0 + H, a.L + 1);
     x.dt = step_dt(a, dt_explicit, loop);
-    x.moff = (uint32_t)(min(x.m, a.Mn + 1)) * 8u;
-    x.outlane = x.m <= a.Mn - 1;
+    const bool central = x.m >= 2 && x.m <= a.Mn - 1;
+    x.outlane = central && lane >= 1 && lane <= 62;
+    x.store = x.outlane || lane == 0 || lane == 63 || x.m > a.Mn + 1;
+    x.st_off = (uint32_t)(kSlotCols * strip + ((lane + 63) & 63)) * 8u;
     x.dz2 = a.dz2;
     x.dz2i = div_recip(a.dz2);
     x.thr = kSpeed ? (float)speed_T(a, loop) * SPEED_CAND_FRAC : 0.0f;
     x.gg1 = (float)(a.gamma * a.gm1);
-    double* halo = halo_lds + (int64_t)wave * 2 * H * HALO_STRIDE;
-    x.halo = halo;
-    // Edge records, 32 rows per pass: lanes 0..31 the west column, lanes
-    // 32..63 the east column (the ghost column past the last strip).
-    for (int j0 = 0; j0 < H; j0 += 32) {
-        const int j = j0 + (lane & 31);
-        const int east = lane >> 5;
-        const int hl = min(x.l0 + j, x.l1 - 1);
-        const int hm = east ? min(66 + 64 * strip, a.Mn + 1) : 1 + 64 * strip;
-        const double* src = a.u_in + at(a, hl, hm);
-        double raw[8];
-        for (int k = 0; k < 8; ++k) raw[k] = src[(int64_t)k * a.fstride];
-        Rec h;
-        row_rec(a, hl, hm, raw, h);
-        if (j < H) halo_put(halo + (east * H + j) * HALO_STRIDE, h);
-    }
-    // The entries are read back by other lanes of this wave only: LDS
-    // operations of one wave complete in order; the fences keep the
-    // compiler from moving the reads above the writes.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     StateIO<kBuf> io;
-    io.init(a, x.moff);
+    // Lanes outside the columns 0 .. M_max+2 read the nearest ghost column.
+    io.init(a, (uint32_t)phys_col(min(max(x.m, 0), a.Mn + 1)) * 8u, x.st_off);
     Rec A, B, C;
     {
         double raw[8];
-        io.load_row(x.l0 - 1, raw);
-        row_rec(a, x.l0 - 1, x.m, raw, A);
-        io.load_row(x.l0, raw);
-        row_rec(a, x.l0, x.m, raw, B);
-        io.load_row(x.l0 + 1, raw);
-        row_rec(a, x.l0 + 1, x.m, raw, C);
+        io.load_row(l0 - 1, raw);
+        row_rec(a, l0 - 1, x.m, raw, A);
+        io.load_row(l0, raw);
+        row_rec(a, l0, x.m, raw, B);
+        io.load_row(l0 + 1, raw);
+        row_rec(a, l0 + 1, x.m, raw, C);
     }
     uint64_t key = 0;
-    int l = x.l0;
+    int l = l0;
     for (; l + 3 <= x.l1; l += 3) {
         march_row<kSpeed, kBuf, Div>(a, x, io, l, A, B, C, key);
         march_row<kSpeed, kBuf, Div>(a, x, io, l + 1, B, C, A, key);
@@ -742,7 +691,7 @@ __global__ void __launch_bounds__(256) lf_bc_ring(StepArgs a, double dt_explicit
         } else if (m >= 2 && m <= Mn - 1) {
             // Uncovered row cell keeps its central value.
             const double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
-            for (int k = 0; k < 8; ++k) u[k] = rb[(int64_t)k * a.pitch + m];
+            for (int k = 0; k < 8; ++k) u[k] = rb[(int64_t)k * a.pitch + phys_col(m)];
         } else {
             // Uncovered column cell: the reference never writes u there, so u
             // (and u0 after the copy) stays 0 whatever the start state held.
@@ -909,16 +858,18 @@ __global__ void __launch_bounds__(256) lf_max_speed(StepArgs a, unsigned long lo
 // Physical fields of the state (10 fields, ghosts 0): the arrays the reference
 // keeps in Fields after a step.
 __global__ void __launch_bounds__(256) lf_phys_fields(StepArgs a, double* out) {
-    const int64_t n = (int64_t)(a.L + 2) * a.pitch;
+    const int cols = a.Mn + 2;
+    const int64_t n = (int64_t)(a.L + 2) * cols;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int l = (int)(i / a.pitch), m = (int)(i % a.pitch);
+        const int l = (int)(i / cols), m = (int)(i % cols);
         double ph[10];
         if (l >= 1 && l <= a.L && m >= 1 && m <= a.Mn) {
             start_phys(a, l, m, ph);
         } else {
             for (int k = 0; k < 10; ++k) ph[k] = 0.0;
         }
-        for (int k = 0; k < 10; ++k) out[(int64_t)k * a.fstride + i] = ph[k];
+        const int64_t j = at(a, l, m);
+        for (int k = 0; k < 10; ++k) out[(int64_t)k * a.fstride + j] = ph[k];
     }
 }
 
@@ -1037,33 +988,28 @@ hipError_t launch_simple(const StepArgs& a, double dt, int loop, int speed, hipS
 
 template <class Div>
 hipError_t launch_tiled(const StepArgs& a, int H, double dt, int loop, int speed, hipStream_t s) {
-    const int nstrips = (a.Mn - 2 + 63) / 64;
+    // Strips holding central columns 2 .. M_max (strip s: columns 62 s .. 62 s + 61).
+    const int nstrips = (a.Mn - 1) / kStripCols + 1;
     const int nchunks = (a.L + H - 1) / H;
     const int ntiles = nstrips * nchunks;
     const unsigned blocks = (unsigned)((ntiles + 3) / 4);
-    const size_t lds = (size_t)4 * 2 * H * HALO_STRIDE * sizeof(double);
     // Buffer addressing needs every row/field offset of a state below 2^31 bytes.
     const bool buf = (double)a.fstride * 8.0 * 8.0 < 2147483648.0;
     if (buf) {
-        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, true, Div>), dim3(blocks), dim3(256), lds, s, a, dt, loop, H, nstrips, ntiles);
-        else hipLaunchKernelGGL((lf_stencil_tiled<false, true, Div>), dim3(blocks), dim3(256), lds, s, a, dt, loop, H, nstrips, ntiles);
+        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+        else hipLaunchKernelGGL((lf_stencil_tiled<false, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
     } else {
-        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, false, Div>), dim3(blocks), dim3(256), lds, s, a, dt, loop, H, nstrips, ntiles);
-        else hipLaunchKernelGGL((lf_stencil_tiled<false, false, Div>), dim3(blocks), dim3(256), lds, s, a, dt, loop, H, nstrips, ntiles);
+        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+        else hipLaunchKernelGGL((lf_stencil_tiled<false, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
     }
     return hipGetLastError();
 }
 
-int auto_rows(const StepArgs& a) {
-    // Many short tiles: about 32 per SIMD, so the last wave of tiles leaves
-    // little of the GPU idle (measured at 6400x2560: H = 8 beats 16 and 32
-    // although each tile reads two extra rows, which mostly hit L2).
-    const int nstrips = (a.Mn - 2 + 63) / 64;
-    const long long target_tiles = 256LL * 4 * 32;
-    long long H = ((long long)a.L * nstrips + target_tiles - 1) / target_tiles;
-    if (H < 8) H = 8;
-    if (H > 32) H = 32;
-    return (int)H;
+int auto_rows(const StepArgs&) {
+    // Short tiles: at 6400x2560 (42 strips) H = 8 measured fastest, ahead of
+    // 24, 67 and 134 (a single round of waves) although each tile reads two
+    // extra rows: the GPU then works on a narrow band of rows at a time.
+    return 8;
 }
 }  // namespace
 
@@ -1073,8 +1019,7 @@ hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt, int
         if (c.fast_div && !a.step1) return launch_simple<DivConst>(a, dt, loop, speed, s);
         return launch_simple<DivExact>(a, dt, loop, speed, s);
     }
-    int H = c.tile_rows > 0 ? c.tile_rows : auto_rows(a);
-    if (H > HALO_MAX_ROWS) H = HALO_MAX_ROWS;
+    const int H = c.tile_rows > 0 ? c.tile_rows : auto_rows(a);
     if (c.fast_div) return launch_tiled<DivConst>(a, H, dt, loop, speed, s);
     return launch_tiled<DivExact>(a, H, dt, loop, speed, s);
 }

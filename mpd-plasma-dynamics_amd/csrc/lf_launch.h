@@ -9,11 +9,19 @@ namespace mpd {
 
 enum StencilKind { STENCIL_SIMPLE = 0, STENCIL_TILED = 1 };
 
-// Offset, in doubles, of column 0 from the start of every pitched field
-// buffer: with a row pitch that is a multiple of 16 doubles and a 256-byte
-// aligned allocation, column 2 (the first central column) and so every
-// 64-column strip of the tiled stencil starts a 128-byte cache line.
-constexpr int kColOff = 14;
+// Row layout of every pitched field buffer.  The columns m = 0 .. M_max + 2
+// of a row are cut into strips of kStripCols consecutive columns; strip s
+// occupies the kSlotCols = 64 doubles (512 bytes, four 128-byte lines) from
+// s * 64, its last two doubles unused.  The tiled stencil gives strip s to one
+// wavefront whose 64 lanes hold columns 62 s - 1 .. 62 s + 62: the two outer
+// lanes read the edge columns of the neighbouring strips in the same row load
+// in which the neighbouring wavefronts read them, and every store covers one
+// whole slot (the two outer lanes write the unused doubles).
+constexpr int kStripCols = 62;
+constexpr int kSlotCols = 64;
+
+__host__ __device__ inline int phys_col(int m) { return (m / kStripCols) * kSlotCols + m % kStripCols; }
+inline int layout_pitch(int cols) { return (cols + kStripCols - 1) / kStripCols * kSlotCols; }
 
 struct StencilCfg {
     int kind = STENCIL_TILED;

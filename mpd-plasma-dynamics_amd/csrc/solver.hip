@@ -208,10 +208,7 @@ struct mpd_solver {
     ~mpd_solver() {
         for (auto& g : graph)
             if (g) (void)hipGraphExecDestroy(g);
-        double* pitched[] = {state[0], state[1], rowbuf, phys_ic, prev};
-        for (double* b : pitched)
-            if (b) (void)hipFree(RawBase(b));
-        double* bufs[] = {partial, rowsum, scratch, geo};
+        double* bufs[] = {state[0], state[1], rowbuf, phys_ic, prev, partial, rowsum, scratch, geo};
         for (double* b : bufs)
             if (b) (void)hipFree(b);
         if (bc) (void)hipFree(bc);
@@ -263,13 +260,20 @@ struct mpd_solver {
         return static_cast<double*>(p);
     }
 
-    // Pitched field buffers: nrow rows of `pitch` doubles, with element (row,
-    // m) at p[row * pitch + m] for the returned p = allocation + kColOff, so
-    // that column 2 (the first central column) starts a 128-byte line.
-    double* AllocPitched(size_t nrow) { return Alloc(nrow * (size_t)pitch) + mpd::kColOff; }
-    static double* RawBase(double* p) { return p ? p - mpd::kColOff : nullptr; }
-    void ZeroPitched(double* p, size_t nrow) {
-        HIPCHK(hipMemset(RawBase(p), 0, nrow * (size_t)pitch * sizeof(double)));
+    // Pitched field buffers: nrow rows of `pitch` doubles in the strip layout
+    // of lf_launch.h (element (row, m) at p[row * pitch + phys_col(m)]).
+    double* AllocPitched(size_t nrow) { return Alloc(nrow * (size_t)pitch); }
+    void ZeroPitched(double* p, size_t nrow) { HIPCHK(hipMemset(p, 0, nrow * (size_t)pitch * sizeof(double))); }
+
+    // Columns m_lo .. m_hi of one row, split at the strip boundaries: calls
+    // f(m, phys_col(m), n) for each run of n consecutive columns.
+    template <class F>
+    static void ForRuns(int m_lo, int m_hi, F f) {
+        for (int m = m_lo; m <= m_hi;) {
+            const int end = std::min(m_hi, (m / mpd::kStripCols) * mpd::kStripCols + mpd::kStripCols - 1);
+            f(m, mpd::phys_col(m), end - m + 1);
+            m = end + 1;
+        }
     }
 
     // Copy nf host arrays (rows x cols) into a pitched device buffer with the
@@ -278,19 +282,27 @@ struct mpd_solver {
         std::vector<double> stage((size_t)rows * pitch, 0.0);
         for (int k = 0; k < nf; ++k) {
             std::fill(stage.begin(), stage.end(), 0.0);
-            for (int l = 1; l <= grid.L; ++l)
-                std::memcpy(&stage[(size_t)l * pitch + mpd::kColOff + 1], src[k] + (size_t)l * cols + 1,
-                            sizeof(double) * grid.Mn);
-            HIPCHK(hipMemcpy(RawBase(dst) + (int64_t)k * fstride, stage.data(), stage.size() * sizeof(double),
+            for (int l = 1; l <= grid.L; ++l) {
+                const double* row = src[k] + (size_t)l * cols;
+                double* out = &stage[(size_t)l * pitch];
+                ForRuns(1, grid.Mn, [&](int m, int pc, int n) { std::memcpy(out + pc, row + m, sizeof(double) * n); });
+            }
+            HIPCHK(hipMemcpy(dst + (int64_t)k * fstride, stage.data(), stage.size() * sizeof(double),
                              hipMemcpyHostToDevice));
         }
     }
 
     void DownloadFields(const double* srcdev, double* const* dst, int nf) {
+        std::vector<double> stage((size_t)rows * pitch);
         for (int k = 0; k < nf; ++k) {
             if (!dst[k]) continue;
-            HIPCHK(hipMemcpy2D(dst[k], sizeof(double) * cols, srcdev + (int64_t)k * fstride, sizeof(double) * pitch,
-                               sizeof(double) * cols, rows, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(stage.data(), srcdev + (int64_t)k * fstride, stage.size() * sizeof(double),
+                             hipMemcpyDeviceToHost));
+            for (int l = 0; l < rows; ++l) {
+                const double* in = &stage[(size_t)l * pitch];
+                double* row = dst[k] + (size_t)l * cols;
+                ForRuns(0, cols - 1, [&](int m, int pc, int n) { std::memcpy(row + m, in + pc, sizeof(double) * n); });
+            }
         }
     }
 
@@ -302,7 +314,7 @@ struct mpd_solver {
         steps_per_graph = cfg.gpu.steps_per_graph;
         rows = grid.rows;
         cols = grid.cols;
-        pitch = (cols + mpd::kColOff + 15) / 16 * 16;
+        pitch = mpd::layout_pitch(cols);
         fstride = (int64_t)rows * pitch;
         mpd::HostFields& f = h.f;
         const MpdBcProgram& hbc = h.bc;
@@ -411,7 +423,7 @@ struct mpd_solver {
         if (!prev) {
             // No snapshot yet: take the current state as the snapshot first.
             prev = AllocPitched((size_t)7 * rows);
-            HIPCHK(hipMemsetAsync(RawBase(prev), 0, (size_t)7 * fstride * sizeof(double), stream));
+            HIPCHK(hipMemsetAsync(prev, 0, (size_t)7 * fstride * sizeof(double), stream));
             HIPCHK(mpd::launch_residual(Args(cur), prev, partial, 0, 1, 0.0, 1, stream));
         }
         HIPCHK(mpd::launch_residual(Args(cur), prev, partial, 0, 1, 0.0, save ? 1 : 0, stream));
@@ -690,11 +702,11 @@ int mpd_download(mpd_solver* s, double* const* u0, double* const* phys) {
                     try {
                         s->DownloadFields(tmp, phys, 10);
                     } catch (...) {
-                        (void)hipFree(mpd_solver::RawBase(tmp));
+                        (void)hipFree(tmp);
                         throw;
                     }
                 }
-                (void)hipFree(mpd_solver::RawBase(tmp));
+                (void)hipFree(tmp);
                 HIPCHK(e);
             }
         }
