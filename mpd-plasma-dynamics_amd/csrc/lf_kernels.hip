@@ -372,26 +372,38 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt
 
 // ---------------------------------------------------- boundary conditions
 
+// Per-thread work area of the expression interpreter in LDS: the symbol
+// table and the evaluation stack, both indexed at run time (in registers
+// they would live in scratch memory).  Laid out [slot][lane] for the
+// 64-thread workgroups of lf_bc_ring, so one slot of a wave is 64 consecutive
+// doubles.
+#define BC_THREADS 64
+struct ExprArea {
+    double* base;  // LDS: (MPD_BS_COUNT + MPD_MAX_STACK) * BC_THREADS doubles
+    int lane;
+    __device__ __forceinline__ double& sym(int i) const { return base[i * BC_THREADS + lane]; }
+    __device__ __forceinline__ double& stk(int i) const { return base[(MPD_BS_COUNT + i) * BC_THREADS + lane]; }
+};
+
 // Evaluate one compiled expression (the exprtk call sites of
-// src/bcs/per_field_bc.cpp:323-358) on the symbol values sym[].
-__device__ double eval_expr(const MpdBcProgram* bc, int off, int len, const double* sym) {
-    double st[MPD_MAX_STACK];
+// src/bcs/per_field_bc.cpp:323-358) on the symbol values in the work area.
+__device__ __noinline__ double eval_expr(const MpdBcProgram* bc, int off, int len, const ExprArea& w) {
     int sp = 0;
     for (int i = 0; i < len; ++i) {
         const MpdInstr in = bc->code[off + i];
         switch (in.op) {
-            case MPD_OP_CONST: st[sp++] = in.val; break;
-            case MPD_OP_SYM: st[sp++] = sym[in.arg]; break;
-            case MPD_OP_NEG: st[sp - 1] = -st[sp - 1]; break;
-            case MPD_OP_ADD: --sp; st[sp - 1] = st[sp - 1] + st[sp]; break;
-            case MPD_OP_SUB: --sp; st[sp - 1] = st[sp - 1] - st[sp]; break;
-            case MPD_OP_MUL: --sp; st[sp - 1] = st[sp - 1] * st[sp]; break;
-            case MPD_OP_DIV: --sp; st[sp - 1] = st[sp - 1] / st[sp]; break;
-            case MPD_OP_MOD: --sp; st[sp - 1] = fmod(st[sp - 1], st[sp]); break;
-            case MPD_OP_POW: --sp; st[sp - 1] = pow(st[sp - 1], st[sp]); break;
-            case MPD_OP_ATAN2: --sp; st[sp - 1] = atan2(st[sp - 1], st[sp]); break;
+            case MPD_OP_CONST: w.stk(sp++) = in.val; break;
+            case MPD_OP_SYM: w.stk(sp++) = w.sym(in.arg); break;
+            case MPD_OP_NEG: w.stk(sp - 1) = -w.stk(sp - 1); break;
+            case MPD_OP_ADD: --sp; w.stk(sp - 1) = w.stk(sp - 1) + w.stk(sp); break;
+            case MPD_OP_SUB: --sp; w.stk(sp - 1) = w.stk(sp - 1) - w.stk(sp); break;
+            case MPD_OP_MUL: --sp; w.stk(sp - 1) = w.stk(sp - 1) * w.stk(sp); break;
+            case MPD_OP_DIV: --sp; w.stk(sp - 1) = w.stk(sp - 1) / w.stk(sp); break;
+            case MPD_OP_MOD: --sp; w.stk(sp - 1) = fmod(w.stk(sp - 1), w.stk(sp)); break;
+            case MPD_OP_POW: --sp; w.stk(sp - 1) = pow(w.stk(sp - 1), w.stk(sp)); break;
+            case MPD_OP_ATAN2: --sp; w.stk(sp - 1) = atan2(w.stk(sp - 1), w.stk(sp)); break;
             case MPD_OP_F1: {
-                const double x = st[sp - 1];
+                const double x = w.stk(sp - 1);
                 double y;
                 switch (in.arg) {
                     case MPD_FN_SIN: y = sin(x); break;
@@ -408,39 +420,47 @@ __device__ double eval_expr(const MpdBcProgram* bc, int off, int len, const doub
                     case MPD_FN_ABS: y = fabs(x); break;
                     default: y = (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0); break;
                 }
-                st[sp - 1] = y;
+                w.stk(sp - 1) = y;
                 break;
             }
             case MPD_OP_CLAMP: {
-                const double lo = st[sp - 3], x = st[sp - 2], hi = st[sp - 1];
+                const double lo = w.stk(sp - 3), x = w.stk(sp - 2), hi = w.stk(sp - 1);
                 sp -= 2;
-                st[sp - 1] = (x < lo) ? lo : ((x > hi) ? hi : x);
+                w.stk(sp - 1) = (x < lo) ? lo : ((x > hi) ? hi : x);
                 break;
             }
             case MPD_OP_MIN:
             case MPD_OP_MAX: {
                 const int n = in.arg;
-                double acc = st[sp - n];
+                double acc = w.stk(sp - n);
                 for (int k = 1; k < n; ++k) {
-                    const double v = st[sp - n + k];
+                    const double v = w.stk(sp - n + k);
                     if (in.op == MPD_OP_MIN) acc = (v < acc) ? v : acc;
                     else acc = (acc < v) ? v : acc;
                 }
                 sp -= n - 1;
-                st[sp - 1] = acc;
+                w.stk(sp - 1) = acc;
                 break;
             }
             default: break;
         }
     }
-    return st[0];
+    return w.stk(0);
 }
 
 // Physical slot of each condition (evaluation order rho, v_z, v_r, v_phi,
 // H_z, H_r, H_phi, e) and of each neighbour symbol (SetNeighbors order rho,
 // v_z, v_r, v_phi, e, H_z, H_r, H_phi).
-__constant__ int kCondSlot[8] = {PH_RHO, PH_VZ, PH_VR, PH_VPHI, PH_HZ, PH_HR, PH_HPHI, PH_E};
-__constant__ int kNbSlot[8] = {PH_RHO, PH_VZ, PH_VR, PH_VPHI, PH_E, PH_HZ, PH_HR, PH_HPHI};
+// (Functions, not tables: indexed by unrolled loop counters they fold to
+// constants and the state arrays stay in registers.)
+__device__ __forceinline__ constexpr int cond_slot(int k) {
+    return k == 0 ? PH_RHO : k == 1 ? PH_VZ : k == 2 ? PH_VR : k == 3 ? PH_VPHI
+         : k == 4 ? PH_HZ : k == 5 ? PH_HR : k == 6 ? PH_HPHI : PH_E;
+}
+__device__ __forceinline__ constexpr int nb_slot(int k) {
+    return k == 0 ? PH_RHO : k == 1 ? PH_VZ : k == 2 ? PH_VR : k == 3 ? PH_VPHI
+         : k == 4 ? PH_E : k == 5 ? PH_HZ : k == 6 ? PH_HR : PH_HPHI;
+}
 
 __device__ __forceinline__ bool seg_covers(const MpdSeg& s, int i) { return i >= s.lo && i <= s.hi; }
 
@@ -465,7 +485,7 @@ __device__ __forceinline__ void face_nb(const StepArgs& a, int f, int l, int m, 
 // physical state st, in order, with neighbour state nb: PerFieldBC::Apply's
 // per-cell physical part (src/bcs/per_field_bc.cpp:297-360, 391-446).
 // Returns the index of the last segment applied, or -1.
-__device__ int face_chain(const StepArgs& a, int f, int l, int m, int seg_end, double* st,
+__device__ __forceinline__ int face_chain(const StepArgs& a, const ExprArea& w, int f, int l, int m, int seg_end, double* st,
                           const double* nb) {
     const MpdBcProgram* bc = a.bc;
     const bool is_l = (f == MPD_L_LO || f == MPD_L_HI);
@@ -475,33 +495,34 @@ __device__ int face_chain(const StepArgs& a, int f, int l, int m, int seg_end, d
         const MpdSeg& g = bc->seg[f][s];
         if (!seg_covers(g, i)) continue;
         last = s;
-        double sym[MPD_BS_COUNT];
         if (g.has_expr) {
             int ln, mn;
             face_nb(a, f, l, m, ln, mn);
-            sym[MPD_BS_GAMMA] = a.gamma;
-            sym[MPD_BS_BETA] = a.beta;
-            sym[MPD_BS_HZ0] = a.H_z0;
-            sym[MPD_BS_R0] = a.r_0;
-            sym[MPD_BS_DZ] = a.dz;
-            sym[MPD_BS_DR] = a.dr[l];
-            sym[MPD_BS_R] = cell_r(a, l, m);
-            sym[MPD_BS_RZ] = cell_rz(a, l, m);
-            sym[MPD_BS_Z] = (double)(l - 1) * a.dz;
-            sym[MPD_BS_RNB] = cell_r(a, ln, mn);
-            sym[MPD_BS_RZNB] = cell_rz(a, ln, mn);
-            for (int k = 0; k < 8; ++k) sym[MPD_BS_RHO_NB + k] = nb[kNbSlot[k]];
+            w.sym(MPD_BS_GAMMA) = a.gamma;
+            w.sym(MPD_BS_BETA) = a.beta;
+            w.sym(MPD_BS_HZ0) = a.H_z0;
+            w.sym(MPD_BS_R0) = a.r_0;
+            w.sym(MPD_BS_DZ) = a.dz;
+            w.sym(MPD_BS_DR) = a.dr[l];
+            w.sym(MPD_BS_R) = cell_r(a, l, m);
+            w.sym(MPD_BS_RZ) = cell_rz(a, l, m);
+            w.sym(MPD_BS_Z) = (double)(l - 1) * a.dz;
+            w.sym(MPD_BS_RNB) = cell_r(a, ln, mn);
+            w.sym(MPD_BS_RZNB) = cell_rz(a, ln, mn);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) w.sym(MPD_BS_RHO_NB + k) = nb[nb_slot(k)];
         }
+#pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int slot = kCondSlot[k];
+            const int slot = cond_slot(k);
             const MpdCond& c = g.c[k];
             double v;
             if (c.type == MPD_NEUMANN) v = nb[slot] + c.value;
             else if (c.type == MPD_DIRICHLET) v = c.value;
-            else if (c.type == MPD_EXPRESSION) v = eval_expr(bc, c.code_off, c.code_len, sym);
+            else if (c.type == MPD_EXPRESSION) v = eval_expr(bc, c.code_off, c.code_len, w);
             else v = st[slot];
             st[slot] = v;
-            if (g.has_expr) sym[MPD_BS_RHO + k] = v;
+            if (g.has_expr) w.sym(MPD_BS_RHO + k) = v;
         }
     }
     return last;
@@ -524,27 +545,27 @@ __device__ __forceinline__ void rebuild_u(const double* ph, double r, double* u)
 // segment s itself when `with_s` (the cell was already visited in s).  Cell
 // (1,1) belongs to L_LO, which ran before; cell (L,1) to L_HI, which runs
 // after.
-__device__ void mlo_state(const StepArgs& a, int l, int s, bool with_s, double* ph) {
+__device__ __forceinline__ void mlo_state(const StepArgs& a, const ExprArea& w, int l, int s, bool with_s, double* ph) {
     double nb[10];
     if (l == 1) {
         s0_phys(a, 1, 1, ph);
         if (face_covers(a.bc, MPD_L_LO, 1)) {
             s0_phys(a, 2, 1, nb);
-            face_chain(a, MPD_L_LO, 1, 1, a.bc->nseg[MPD_L_LO], ph, nb);
+            face_chain(a, w, MPD_L_LO, 1, 1, a.bc->nseg[MPD_L_LO], ph, nb);
         }
         return;
     }
     s0_phys(a, l, 1, ph);
     if (l == a.L) return;
     s0_phys(a, l, 2, nb);
-    face_chain(a, MPD_M_LO, l, 1, with_s ? s + 1 : s, ph, nb);
+    face_chain(a, w, MPD_M_LO, l, 1, with_s ? s + 1 : s, ph, nb);
 }
 
 // PerFieldBC::AxisLfU1/U2/U5/U7, src/bcs/per_field_bc.cpp:490-566, at (l, 1).
 // own: the cell's physical values after the segment; lo/hi: the states of
 // (l-1, 1) and (l+1, 1); c2: the state of (l, 2); sp: start-of-step physical
 // values at (l-1,1), (l,1), (l+1,1) (p and P are never written by the BCs).
-__device__ void axis_overrides(const StepArgs& a, const MpdSeg& g, int l, double dt, const double* own,
+__device__ __forceinline__ void axis_overrides(const StepArgs& a, const MpdSeg& g, int l, double dt, const double* own,
                                const double* lo, const double* hi, const double* c2, const double* sp_lo,
                                const double* sp_c, const double* sp_hi, double* u) {
     const int m = 1;
@@ -591,7 +612,7 @@ __device__ void axis_overrides(const StepArgs& a, const MpdSeg& g, int l, double
 
 // Final conserved values of boundary cell (l, m) written by face f (the last
 // face in L_LO, M_HI, M_LO, L_HI order that covers it).
-__device__ void bc_final(const StepArgs& a, int f, int l, int m, double dt, double* u) {
+__device__ __forceinline__ void bc_final(const StepArgs& a, const ExprArea& w, int f, int l, int m, double dt, double* u) {
     double st[10], nb[10];
     // Own state before the face: only AxisLF conditions (M_LO only) read it.
     s0_phys(a, l, m, st);
@@ -603,24 +624,24 @@ __device__ void bc_final(const StepArgs& a, int f, int l, int m, double dt, doub
     if (f == MPD_M_HI && ln == 1 && face_covers(a.bc, MPD_L_LO, mn)) {
         double nb2[10];
         s0_phys(a, 2, mn, nb2);
-        face_chain(a, MPD_L_LO, 1, mn, a.bc->nseg[MPD_L_LO], nb, nb2);
+        face_chain(a, w, MPD_L_LO, 1, mn, a.bc->nseg[MPD_L_LO], nb, nb2);
     } else if (f == MPD_L_HI && mn == 1 && face_covers(a.bc, MPD_M_LO, ln)) {
         double nb2[10];
         s0_phys(a, ln, 2, nb2);
-        face_chain(a, MPD_M_LO, ln, 1, a.bc->nseg[MPD_M_LO], nb, nb2);
+        face_chain(a, w, MPD_M_LO, ln, 1, a.bc->nseg[MPD_M_LO], nb, nb2);
     } else if (f == MPD_L_HI && mn == a.Mn && face_covers(a.bc, MPD_M_HI, ln)) {
         double nb2[10];
         s0_phys(a, ln, a.Mn - 1, nb2);
-        face_chain(a, MPD_M_HI, ln, a.Mn, a.bc->nseg[MPD_M_HI], nb, nb2);
+        face_chain(a, w, MPD_M_HI, ln, a.Mn, a.bc->nseg[MPD_M_HI], nb, nb2);
     }
-    const int last = face_chain(a, f, l, m, a.bc->nseg[f], st, nb);
+    const int last = face_chain(a, w, f, l, m, a.bc->nseg[f], st, nb);
     const double r = cell_r(a, l, m);
     rebuild_u(st, r, u);
     if (f == MPD_M_LO && last >= 0 && a.bc->seg[f][last].has_axis) {
         const MpdSeg& g = a.bc->seg[f][last];
         double lo[10], hi[10], sp_lo[10], sp_c[10], sp_hi[10];
-        mlo_state(a, l - 1, last, seg_covers(g, l - 1), lo);
-        mlo_state(a, l + 1, last, false, hi);
+        mlo_state(a, w, l - 1, last, seg_covers(g, l - 1), lo);
+        mlo_state(a, w, l + 1, last, false, hi);
         start_phys(a, l - 1, 1, sp_lo);
         start_phys(a, l, 1, sp_c);
         start_phys(a, l + 1, 1, sp_hi);
@@ -657,8 +678,22 @@ __device__ void finalize_step(const StepArgs& a) {
 // One thread per boundary cell: row 1, row L, column 1 (l = 2..L-1), column
 // Mn (l = 2..L-1).  Writes each cell's final conserved values, adds its wave
 // speed to the running max, and the last block finishes the step.
-__global__ void __launch_bounds__(256) lf_bc_ring(StepArgs a, double dt_explicit, int loop, int speed) {
-    if (loop && a.st->done) return;
+__global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double dt_explicit, int loop, int speed) {
+    if (loop && a_in.st->done) return;
+    // The BC program (segments and the expression code in use) is read by
+    // every thread many times over in dependent chains: copy it to LDS once.
+    __shared__ uint64_t prog[sizeof(MpdBcProgram) / 8];
+    __shared__ double area[(MPD_BS_COUNT + MPD_MAX_STACK) * BC_THREADS];
+    {
+        const uint64_t* src = reinterpret_cast<const uint64_t*>(a_in.bc);
+        const int ncode = a_in.bc->ncode;
+        const int nw = (int)((offsetof(MpdBcProgram, code) + (size_t)ncode * sizeof(MpdInstr)) / 8);
+        for (int i = (int)threadIdx.x; i < nw; i += (int)blockDim.x) prog[i] = src[i];
+        __syncthreads();
+    }
+    StepArgs a = a_in;
+    a.bc = reinterpret_cast<const MpdBcProgram*>(prog);
+    const ExprArea w{area, (int)threadIdx.x};
     const double dt = step_dt(a, dt_explicit, loop);
     const int L = a.L, Mn = a.Mn;
     const int nring = 2 * Mn + 2 * (L - 2);
@@ -687,7 +722,7 @@ __global__ void __launch_bounds__(256) lf_bc_ring(StepArgs a, double dt_explicit
         }
         double u[8];
         if (f >= 0) {
-            bc_final(a, f, l, m, dt, u);
+            bc_final(a, w, f, l, m, dt, u);
         } else if (m >= 2 && m <= Mn - 1) {
             // Uncovered row cell keeps its central value.
             const double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
@@ -1026,7 +1061,8 @@ hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt, int
 
 hipError_t launch_bc(const StepArgs& a, double dt, int loop, int speed, hipStream_t s) {
     const int nring = 2 * a.Mn + 2 * (a.L - 2);
-    hipLaunchKernelGGL(lf_bc_ring, dim3((unsigned)((nring + 255) / 256)), dim3(256), 0, s, a, dt, loop, speed);
+    hipLaunchKernelGGL(lf_bc_ring, dim3((unsigned)((nring + BC_THREADS - 1) / BC_THREADS)), dim3(BC_THREADS), 0, s, a,
+                       dt, loop, speed);
     return hipGetLastError();
 }
 

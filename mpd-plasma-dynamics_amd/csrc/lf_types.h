@@ -36,7 +36,7 @@ struct MpdInstr {
 };
 
 #define MPD_MAX_SEGS_PER_FACE 16
-#define MPD_MAX_CODE 2048
+#define MPD_MAX_CODE 1024
 #define MPD_MAX_STACK 16
 
 // One field's condition.  Expression code lives in MpdBcProgram::code.
