@@ -372,22 +372,60 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt
 
 // ---------------------------------------------------- boundary conditions
 
-// Per-thread work area of the expression interpreter in LDS: the symbol
-// table and the evaluation stack, both indexed at run time (in registers
-// they would live in scratch memory).  Laid out [slot][lane] for the
-// 64-thread workgroups of lf_bc_ring, so one slot of a wave is 64 consecutive
-// doubles.
+// Per-thread work area in LDS for the boundary evaluation: the expression
+// symbol table and evaluation stack, and the state and neighbour state of
+// the face chain being applied -- everything the condition loop indexes at
+// run time (in registers these would be private arrays in scratch memory).
+// Laid out [slot][lane] for the 64-thread workgroups of lf_bc_ring, so one
+// slot of a wave is 64 consecutive doubles.
 #define BC_THREADS 64
-struct ExprArea {
-    double* base;  // LDS: (MPD_BS_COUNT + MPD_MAX_STACK) * BC_THREADS doubles
+enum { WA_SYM = 0, WA_STK = WA_SYM + MPD_BS_COUNT, WA_CUR = WA_STK + MPD_MAX_STACK, WA_NBR = WA_CUR + 10,
+       WA_N = WA_NBR + 10 };
+struct WorkArea {
+    double* base;  // LDS: WA_N * BC_THREADS doubles
     int lane;
-    __device__ __forceinline__ double& sym(int i) const { return base[i * BC_THREADS + lane]; }
-    __device__ __forceinline__ double& stk(int i) const { return base[(MPD_BS_COUNT + i) * BC_THREADS + lane]; }
+    __device__ __forceinline__ double& at(int i) const { return base[i * BC_THREADS + lane]; }
+    __device__ __forceinline__ double& sym(int i) const { return at(WA_SYM + i); }
+    __device__ __forceinline__ double& stk(int i) const { return at(WA_STK + i); }
+    __device__ __forceinline__ double& cur(int i) const { return at(WA_CUR + i); }
+    __device__ __forceinline__ double& nbr(int i) const { return at(WA_NBR + i); }
+    // Copies between a register array of 10 physical values and an area.
+    __device__ __forceinline__ void put(int area, const double* v) const {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) at(area + k) = v[k];
+    }
+    __device__ __forceinline__ void get(int area, double* v) const {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) v[k] = at(area + k);
+    }
 };
+
+// The rarely used operations of the expression code, out of line so that
+// the interpreter stays small where it is inlined.
+__device__ __noinline__ double bc_fn1(int fn, double x) {
+    switch (fn) {
+        case MPD_FN_SIN: return sin(x);
+        case MPD_FN_COS: return cos(x);
+        case MPD_FN_TAN: return tan(x);
+        case MPD_FN_ASIN: return asin(x);
+        case MPD_FN_ACOS: return acos(x);
+        case MPD_FN_ATAN: return atan(x);
+        case MPD_FN_EXP: return exp(x);
+        case MPD_FN_LOG: return log(x);
+        case MPD_FN_LOG2: return log2(x);
+        default: return log10(x);
+    }
+}
+
+__device__ __noinline__ double bc_fn2(int op, double x, double y) {
+    if (op == MPD_OP_MOD) return fmod(x, y);
+    if (op == MPD_OP_POW) return pow(x, y);
+    return atan2(x, y);
+}
 
 // Evaluate one compiled expression (the exprtk call sites of
 // src/bcs/per_field_bc.cpp:323-358) on the symbol values in the work area.
-__device__ __noinline__ double eval_expr(const MpdBcProgram* bc, int off, int len, const ExprArea& w) {
+__device__ __forceinline__ double eval_expr(const MpdBcProgram* bc, int off, int len, const WorkArea& w) {
     int sp = 0;
     for (int i = 0; i < len; ++i) {
         const MpdInstr in = bc->code[off + i];
@@ -399,27 +437,16 @@ __device__ __noinline__ double eval_expr(const MpdBcProgram* bc, int off, int le
             case MPD_OP_SUB: --sp; w.stk(sp - 1) = w.stk(sp - 1) - w.stk(sp); break;
             case MPD_OP_MUL: --sp; w.stk(sp - 1) = w.stk(sp - 1) * w.stk(sp); break;
             case MPD_OP_DIV: --sp; w.stk(sp - 1) = w.stk(sp - 1) / w.stk(sp); break;
-            case MPD_OP_MOD: --sp; w.stk(sp - 1) = fmod(w.stk(sp - 1), w.stk(sp)); break;
-            case MPD_OP_POW: --sp; w.stk(sp - 1) = pow(w.stk(sp - 1), w.stk(sp)); break;
-            case MPD_OP_ATAN2: --sp; w.stk(sp - 1) = atan2(w.stk(sp - 1), w.stk(sp)); break;
+            case MPD_OP_MOD:
+            case MPD_OP_POW:
+            case MPD_OP_ATAN2: --sp; w.stk(sp - 1) = bc_fn2(in.op, w.stk(sp - 1), w.stk(sp)); break;
             case MPD_OP_F1: {
                 const double x = w.stk(sp - 1);
                 double y;
-                switch (in.arg) {
-                    case MPD_FN_SIN: y = sin(x); break;
-                    case MPD_FN_COS: y = cos(x); break;
-                    case MPD_FN_TAN: y = tan(x); break;
-                    case MPD_FN_ASIN: y = asin(x); break;
-                    case MPD_FN_ACOS: y = acos(x); break;
-                    case MPD_FN_ATAN: y = atan(x); break;
-                    case MPD_FN_EXP: y = exp(x); break;
-                    case MPD_FN_LOG: y = log(x); break;
-                    case MPD_FN_LOG2: y = log2(x); break;
-                    case MPD_FN_LOG10: y = log10(x); break;
-                    case MPD_FN_SQRT: y = sqrt(x); break;
-                    case MPD_FN_ABS: y = fabs(x); break;
-                    default: y = (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0); break;
-                }
+                if (in.arg == MPD_FN_SQRT) y = sqrt(x);
+                else if (in.arg == MPD_FN_ABS) y = fabs(x);
+                else if (in.arg == MPD_FN_SIGN) y = (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0);
+                else y = bc_fn1(in.arg, x);
                 w.stk(sp - 1) = y;
                 break;
             }
@@ -482,11 +509,12 @@ __device__ __forceinline__ void face_nb(const StepArgs& a, int f, int l, int m, 
 }
 
 // Apply the segments [0, seg_end) of face f that cover cell (l, m) to the
-// physical state st, in order, with neighbour state nb: PerFieldBC::Apply's
-// per-cell physical part (src/bcs/per_field_bc.cpp:297-360, 391-446).
-// Returns the index of the last segment applied, or -1.
-__device__ __forceinline__ int face_chain(const StepArgs& a, const ExprArea& w, int f, int l, int m, int seg_end, double* st,
-                          const double* nb) {
+// physical state in the work area's `cur` slots, with the neighbour state in
+// its `nbr` slots: PerFieldBC::Apply's per-cell physical part
+// (src/bcs/per_field_bc.cpp:297-360, 391-446).  Returns the index of the last
+// segment applied, or -1.  The condition loop stays rolled: one copy of the
+// interpreter per call site.
+__device__ __forceinline__ int face_chain(const StepArgs& a, const WorkArea& w, int f, int l, int m, int seg_end) {
     const MpdBcProgram* bc = a.bc;
     const bool is_l = (f == MPD_L_LO || f == MPD_L_HI);
     const int i = is_l ? m : l;
@@ -510,18 +538,18 @@ __device__ __forceinline__ int face_chain(const StepArgs& a, const ExprArea& w, 
             w.sym(MPD_BS_RNB) = cell_r(a, ln, mn);
             w.sym(MPD_BS_RZNB) = cell_rz(a, ln, mn);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) w.sym(MPD_BS_RHO_NB + k) = nb[nb_slot(k)];
+            for (int k = 0; k < 8; ++k) w.sym(MPD_BS_RHO_NB + k) = w.nbr(nb_slot(k));
         }
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < 8; ++k) {
             const int slot = cond_slot(k);
             const MpdCond& c = g.c[k];
             double v;
-            if (c.type == MPD_NEUMANN) v = nb[slot] + c.value;
+            if (c.type == MPD_NEUMANN) v = w.nbr(slot) + c.value;
             else if (c.type == MPD_DIRICHLET) v = c.value;
             else if (c.type == MPD_EXPRESSION) v = eval_expr(bc, c.code_off, c.code_len, w);
-            else v = st[slot];
-            st[slot] = v;
+            else v = w.cur(slot);
+            w.cur(slot) = v;
             if (g.has_expr) w.sym(MPD_BS_RHO + k) = v;
         }
     }
@@ -540,25 +568,34 @@ __device__ __forceinline__ void rebuild_u(const double* ph, double r, double* u)
     u[7] = ph[PH_HR] * r;
 }
 
-// State of boundary cell (l, m) as the M_LO face sees it while applying its
+// State of boundary cell (l, 1) as the M_LO face sees it while applying its
 // segment s (serial order, l ascending): the M_LO segments before s, plus
 // segment s itself when `with_s` (the cell was already visited in s).  Cell
 // (1,1) belongs to L_LO, which ran before; cell (L,1) to L_HI, which runs
 // after.
-__device__ __forceinline__ void mlo_state(const StepArgs& a, const ExprArea& w, int l, int s, bool with_s, double* ph) {
-    double nb[10];
-    if (l == 1) {
-        s0_phys(a, 1, 1, ph);
-        if (face_covers(a.bc, MPD_L_LO, 1)) {
-            s0_phys(a, 2, 1, nb);
-            face_chain(a, w, MPD_L_LO, 1, 1, a.bc->nseg[MPD_L_LO], ph, nb);
-        }
-        return;
-    }
+__device__ __forceinline__ void mlo_state(const StepArgs& a, const WorkArea& w, int l, int s, bool with_s,
+                                          double* ph) {
     s0_phys(a, l, 1, ph);
-    if (l == a.L) return;
-    s0_phys(a, l, 2, nb);
-    face_chain(a, w, MPD_M_LO, l, 1, with_s ? s + 1 : s, ph, nb);
+    int cf = -1, ce = 0, nl = l, nm = 2;
+    if (l == 1) {
+        if (face_covers(a.bc, MPD_L_LO, 1)) {
+            cf = MPD_L_LO;
+            ce = a.bc->nseg[MPD_L_LO];
+            nl = 2;
+            nm = 1;
+        }
+    } else if (l != a.L) {
+        cf = MPD_M_LO;
+        ce = with_s ? s + 1 : s;
+    }
+    if (cf >= 0) {
+        double nb[10];
+        s0_phys(a, nl, nm, nb);
+        w.put(WA_CUR, ph);
+        w.put(WA_NBR, nb);
+        face_chain(a, w, cf, l, 1, ce);
+        w.get(WA_CUR, ph);
+    }
 }
 
 // PerFieldBC::AxisLfU1/U2/U5/U7, src/bcs/per_field_bc.cpp:490-566, at (l, 1).
@@ -612,7 +649,8 @@ __device__ __forceinline__ void axis_overrides(const StepArgs& a, const MpdSeg& 
 
 // Final conserved values of boundary cell (l, m) written by face f (the last
 // face in L_LO, M_HI, M_LO, L_HI order that covers it).
-__device__ __forceinline__ void bc_final(const StepArgs& a, const ExprArea& w, int f, int l, int m, double dt, double* u) {
+__device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, int f, int l, int m, double dt,
+                                         double* u) {
     double st[10], nb[10];
     // Own state before the face: only AxisLF conditions (M_LO only) read it.
     s0_phys(a, l, m, st);
@@ -621,20 +659,26 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const ExprArea& w, i
     int ln, mn;
     face_nb(a, f, l, m, ln, mn);
     s0_phys(a, ln, mn, nb);
+    int pf = -1, pl = 0, pm = 0, ql = 0, qm = 0;  // earlier face at the neighbour, its cell and neighbour
     if (f == MPD_M_HI && ln == 1 && face_covers(a.bc, MPD_L_LO, mn)) {
-        double nb2[10];
-        s0_phys(a, 2, mn, nb2);
-        face_chain(a, w, MPD_L_LO, 1, mn, a.bc->nseg[MPD_L_LO], nb, nb2);
+        pf = MPD_L_LO; pl = 1; pm = mn; ql = 2; qm = mn;
     } else if (f == MPD_L_HI && mn == 1 && face_covers(a.bc, MPD_M_LO, ln)) {
-        double nb2[10];
-        s0_phys(a, ln, 2, nb2);
-        face_chain(a, w, MPD_M_LO, ln, 1, a.bc->nseg[MPD_M_LO], nb, nb2);
+        pf = MPD_M_LO; pl = ln; pm = 1; ql = ln; qm = 2;
     } else if (f == MPD_L_HI && mn == a.Mn && face_covers(a.bc, MPD_M_HI, ln)) {
-        double nb2[10];
-        s0_phys(a, ln, a.Mn - 1, nb2);
-        face_chain(a, w, MPD_M_HI, ln, a.Mn, a.bc->nseg[MPD_M_HI], nb, nb2);
+        pf = MPD_M_HI; pl = ln; pm = a.Mn; ql = ln; qm = a.Mn - 1;
     }
-    const int last = face_chain(a, w, f, l, m, a.bc->nseg[f], st, nb);
+    if (pf >= 0) {
+        double nb2[10];
+        s0_phys(a, ql, qm, nb2);
+        w.put(WA_CUR, nb);
+        w.put(WA_NBR, nb2);
+        face_chain(a, w, pf, pl, pm, a.bc->nseg[pf]);
+        w.get(WA_CUR, nb);
+    }
+    w.put(WA_CUR, st);
+    w.put(WA_NBR, nb);
+    const int last = face_chain(a, w, f, l, m, a.bc->nseg[f]);
+    w.get(WA_CUR, st);
     const double r = cell_r(a, l, m);
     rebuild_u(st, r, u);
     if (f == MPD_M_LO && last >= 0 && a.bc->seg[f][last].has_axis) {
@@ -683,7 +727,7 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
     // The BC program (segments and the expression code in use) is read by
     // every thread many times over in dependent chains: copy it to LDS once.
     __shared__ uint64_t prog[sizeof(MpdBcProgram) / 8];
-    __shared__ double area[(MPD_BS_COUNT + MPD_MAX_STACK) * BC_THREADS];
+    __shared__ double area[WA_N * BC_THREADS];
     {
         const uint64_t* src = reinterpret_cast<const uint64_t*>(a_in.bc);
         const int ncode = a_in.bc->ncode;
@@ -693,7 +737,7 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
     }
     StepArgs a = a_in;
     a.bc = reinterpret_cast<const MpdBcProgram*>(prog);
-    const ExprArea w{area, (int)threadIdx.x};
+    const WorkArea w{area, (int)threadIdx.x};
     const double dt = step_dt(a, dt_explicit, loop);
     const int L = a.L, Mn = a.Mn;
     const int nring = 2 * Mn + 2 * (L - 2);
