@@ -37,7 +37,8 @@ typedef struct {
     int steps;       /* steps taken by this call */
     int total_steps; /* steps taken since creation */
     int converged;   /* SolutionChange fell below convergence.threshold */
-    int done;        /* the loop condition is false (t >= T, converged, or step cap) */
+    int done;        /* the time loop has ended: t >= T, converged, or time.max_steps reached
+                        (0 when only this call's max_steps stopped it) */
 } mpd_run_result;
 
 /* Build a solver from a config.yaml path / text: SimConfig::Load
@@ -104,6 +105,22 @@ int mpd_config_check(const char* yaml_text, int* L_max, int* M_max);
  * src/fields.cpp:40-55) into caller arrays of rows x cols (either may be
  * NULL). */
 int mpd_initial_state(const char* yaml_text, double* const* u0, double* const* phys);
+
+/* Physical values of cell (l, m) (ghost ring included, 0 there) in the
+ * Fields member order rho, v_z, v_r, v_phi, e, p, P, H_z, H_r, H_phi: what
+ * the driver's checkpoint line reads (src/main.cpp:259-264). */
+int mpd_probe(mpd_solver* s, int l, int m, double out10[10]);
+
+/* IOManager::WriteFrame's VTK file (src/io_manager.cpp:218-315) of the
+ * current state: legacy structured grid, binary (big-endian), points
+ * (z, r, 0) over (L_max+1) x (M_max+1), arrays Rho, Energy, MPI_Rank,
+ * Hphi_times_r, Velocity, MagneticField. */
+int mpd_write_vtk(mpd_solver* s, const char* path);
+
+/* Host-only: the same file from 10 host physical arrays (rows x cols, Fields
+ * member order) on the grid of a config; binary != 0 for the binary form,
+ * 0 for ASCII. */
+int mpd_write_vtk_fields(const char* yaml_text, const double* const* phys, const char* path, int binary);
 
 /* Kernel selection: key "kernel" (0 = simple, 1 = tiled), "fast_div" (0/1),
  * "tile_rows", "steps_per_graph". */

@@ -1043,6 +1043,19 @@ __global__ void lf_recip_table(const double* d, double* y, int n) {
     if (i < n) y[i] = div_recip(2 * d[i]);
 }
 
+// Physical values of one cell, 0 on the ghost ring: what the driver's
+// checkpoint line reads from Fields (src/main.cpp:259-264).
+__global__ void lf_probe(StepArgs a, int l, int m, double* out) {
+    if (threadIdx.x != 0) return;
+    double ph[10];
+    if (l >= 1 && l <= a.L && m >= 1 && m <= a.Mn) {
+        start_phys(a, l, m, ph);
+    } else {
+        for (int k = 0; k < 10; ++k) ph[k] = 0.0;
+    }
+    for (int k = 0; k < 10; ++k) out[k] = ph[k];
+}
+
 // Fill kernel (used instead of memset nodes inside captured graphs).
 __global__ void lf_fill(double* p, int64_t n, double v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -1150,6 +1163,11 @@ hipError_t launch_outlet(const StepArgs& a, int M_max, double* out, hipStream_t 
 
 hipError_t launch_radial_sym(const StepArgs& a, unsigned long long* out, hipStream_t s) {
     hipLaunchKernelGGL(lf_radial_sym, dim3(1024), dim3(256), 0, s, a, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe(const StepArgs& a, int l, int m, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(lf_probe, dim3(1), dim3(64), 0, s, a, l, m, out);
     return hipGetLastError();
 }
 

@@ -48,6 +48,8 @@ hipError_t launch_integrals(const StepArgs& a, double* rowsum, double* out, hipS
 hipError_t launch_outlet(const StepArgs& a, int M_max, double* out, hipStream_t s);
 hipError_t launch_radial_sym(const StepArgs& a, unsigned long long* out, hipStream_t s);
 hipError_t launch_fill(double* p, long long n, double v, hipStream_t s);
+// Physical values of cell (l, m) into out[0..9].
+hipError_t launch_probe(const StepArgs& a, int l, int m, double* out, hipStream_t s);
 // y[i] = div_recip(2 d[i]) for i < n.
 hipError_t launch_recip_table(const double* d, double* y, int n, hipStream_t s);
 

@@ -16,6 +16,7 @@
 #include "../host/config.h"
 #include "../host/expr.h"
 #include "../host/setup.h"
+#include "../host/vtk.h"
 #include "lf_launch.h"
 
 using mpd::StepArgs;
@@ -151,20 +152,25 @@ struct HostSetup {
         if (cfg.L_max < 3 || cfg.M_max < 2) throw std::runtime_error("grid: L_max must be >= 3 and M_max >= 2");
         if (cfg.convergence_threshold > 0.0 && cfg.check_frequency <= 0)
             throw std::runtime_error("convergence: check_frequency must be positive");
-        if (!cfg.initial_conditions.vtk_file.empty())
-            throw std::runtime_error("initial_conditions.vtk_file: the VTK restart IC is not part of this library");
         if (cfg.gpu.kernel == "simple") scfg.kind = mpd::STENCIL_SIMPLE;
         else if (cfg.gpu.kernel == "tiled") scfg.kind = mpd::STENCIL_TILED;
         else throw std::runtime_error("gpu.kernel must be 'tiled' or 'simple'");
         scfg.fast_div = cfg.gpu.fast_div;
         scfg.tile_rows = cfg.gpu.tile_rows;
         auto geom = mpd::GeometryRegistry::Instance().Create(cfg.geometry.type, cfg.geometry.params);
-        mpd::ExpressionIC ic(cfg.initial_conditions.has_params ? &cfg.initial_conditions.params : nullptr,
-                             cfg.rho_noise_amplitude, cfg.rho_noise_seed);
+        // Initial condition: VTK restart when initial_conditions.vtk_file is
+        // set, the expression IC otherwise (src/main.cpp:117-136).
+        std::unique_ptr<mpd::IInitialCondition> ic;
+        if (!cfg.initial_conditions.vtk_file.empty())
+            ic = std::make_unique<mpd::VtkIC>(cfg.initial_conditions.vtk_file, cfg.initial_conditions.vtk_exact_z);
+        else
+            ic = std::make_unique<mpd::ExpressionIC>(
+                cfg.initial_conditions.has_params ? &cfg.initial_conditions.params : nullptr,
+                cfg.rho_noise_amplitude, cfg.rho_noise_seed);
         grid.Build(cfg, *geom);
         CompileBc(cfg, bc);
         f.Allocate(grid.rows, grid.cols);
-        ic.Apply(f, grid, cfg, 0);
+        ic->Apply(f, grid, cfg, 0);
         f.InitConservative(grid);
     }
 };
@@ -304,6 +310,28 @@ struct mpd_solver {
                 ForRuns(0, cols - 1, [&](int m, int pc, int n) { std::memcpy(row + m, in + pc, sizeof(double) * n); });
             }
         }
+    }
+
+    // The physical arrays Fields holds after the last step (or the IC before
+    // the first), rows x cols each.
+    void DownloadPhys(double* const* phys) {
+        if (step1) {
+            DownloadFields(phys_ic, phys, 10);
+            return;
+        }
+        double* tmp = AllocPitched((size_t)10 * rows);
+        const hipError_t e = mpd::launch_phys_fields(Args(cur), tmp, stream);
+        if (e == hipSuccess) {
+            (void)hipStreamSynchronize(stream);
+            try {
+                DownloadFields(tmp, phys, 10);
+            } catch (...) {
+                (void)hipFree(tmp);
+                throw;
+            }
+        }
+        (void)hipFree(tmp);
+        HIPCHK(e);
     }
 
     void Init(const std::string& src, bool is_file) {
@@ -541,7 +569,8 @@ struct mpd_solver {
             out->steps = step - step0;
             out->total_steps = step;
             out->converged = converged ? 1 : 0;
-            out->done = 1;
+            // The loop's own end (src/main.cpp:219), not this call's step cap.
+            out->done = (!(t < cfg.T) || converged || (cfg.max_steps > 0 && step >= cfg.max_steps)) ? 1 : 0;
         }
     }
 
@@ -691,25 +720,7 @@ int mpd_download(mpd_solver* s, double* const* u0, double* const* phys) {
         Need(s);
         HIPCHK(hipStreamSynchronize(s->stream));
         if (u0) s->DownloadFields(s->state[s->cur], u0, 8);
-        if (phys) {
-            if (s->step1) {
-                s->DownloadFields(s->phys_ic, phys, 10);
-            } else {
-                double* tmp = s->AllocPitched((size_t)10 * s->rows);
-                const hipError_t e = mpd::launch_phys_fields(s->Args(s->cur), tmp, s->stream);
-                if (e == hipSuccess) {
-                    (void)hipStreamSynchronize(s->stream);
-                    try {
-                        s->DownloadFields(tmp, phys, 10);
-                    } catch (...) {
-                        (void)hipFree(tmp);
-                        throw;
-                    }
-                }
-                (void)hipFree(tmp);
-                HIPCHK(e);
-            }
-        }
+        if (phys) s->DownloadPhys(phys);
     });
 }
 
@@ -820,6 +831,47 @@ int mpd_initial_state(const char* yaml_text, double* const* u0, double* const* p
             if (u0 && u0[k]) std::memcpy(u0[k], h.f.u0[k].data(), n * sizeof(double));
         for (int k = 0; k < 10; ++k)
             if (phys && phys[k]) std::memcpy(phys[k], h.f.phys[k].data(), n * sizeof(double));
+    });
+}
+
+int mpd_probe(mpd_solver* s, int l, int m, double out10[10]) {
+    return Guard([&] {
+        Need(s);
+        Need(out10);
+        if (l < 0 || l >= s->rows || m < 0 || m >= s->cols) throw ArgError("mpd_probe: cell outside the grid");
+        HIPCHK(mpd::launch_probe(s->Args(s->cur), l, m, s->scratch, s->stream));
+        HIPCHK(hipMemcpyAsync(out10, s->scratch, 10 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int mpd_write_vtk(mpd_solver* s, const char* path) {
+    return Guard([&] {
+        Need(s);
+        Need(path);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        std::vector<double> ph[10];
+        double* pp[10];
+        for (int k = 0; k < 10; ++k) {
+            ph[k].assign((size_t)s->rows * s->cols, 0.0);
+            pp[k] = ph[k].data();
+        }
+        s->DownloadPhys(pp);
+        mpd::WriteVtk(path, mpd::MakeFrame(s->cfg, s->grid, ph));
+    });
+}
+
+int mpd_write_vtk_fields(const char* yaml_text, const double* const* phys, const char* path, int binary) {
+    return Guard([&] {
+        Need(yaml_text);
+        Need(phys);
+        Need(path);
+        for (int k = 0; k < 10; ++k) Need(phys[k]);
+        HostSetup h(yaml_text);
+        const size_t n = (size_t)h.grid.rows * h.grid.cols;
+        std::vector<double> ph[10];
+        for (int k = 0; k < 10; ++k) ph[k].assign(phys[k], phys[k] + n);
+        mpd::WriteVtk(path, mpd::MakeFrame(h.cfg, h.grid, ph), binary != 0);
     });
 }
 

@@ -213,6 +213,8 @@ void SimConfig::LoadNode(const YNode& cfg) {
                 throw std::runtime_error(
                     "config: 'initial_conditions.vtk_file' must be a scalar file-path string.");
             initial_conditions.vtk_file = vf->scalar;
+            // Extension: map axial cells exactly onto a same-size frame (VtkIC).
+            if (const YNode* x = n->Get("vtk_exact_z")) initial_conditions.vtk_exact_z = x->AsBool();
         } else {
             initial_conditions.params = *n;
             initial_conditions.has_params = true;

@@ -40,6 +40,7 @@ struct GeometryConfig {
 
 struct ICConfig {
     std::string vtk_file;
+    bool vtk_exact_z = false;  // VtkIC axial mapping (see host/vtk.h)
     YNode params;  // the initial_conditions node (expression IC)
     bool has_params = false;
 };

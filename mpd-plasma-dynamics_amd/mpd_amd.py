@@ -39,7 +39,8 @@ class RunResult(C.Structure):
 EXPORTS = ["mpd_create_from_file", "mpd_create_from_string", "mpd_destroy", "mpd_last_error", "mpd_dims",
            "mpd_upload_state", "mpd_download", "mpd_advance", "mpd_compute_dt", "mpd_max_wave_speed",
            "mpd_solution_change", "mpd_run", "mpd_integrals", "mpd_outlet", "mpd_radial_symmetry", "mpd_grid",
-           "mpd_config_check", "mpd_initial_state", "mpd_set_option", "mpd_time_steps"]
+           "mpd_config_check", "mpd_initial_state", "mpd_set_option", "mpd_time_steps", "mpd_probe",
+           "mpd_write_vtk", "mpd_write_vtk_fields"]
 
 _lib = None
 _DP = C.POINTER(C.c_double)
@@ -106,6 +107,12 @@ def initial_state(config):
     pa = [np.zeros((rows, cols)) for _ in range(10)]
     _check(load().mpd_initial_state(config.encode(), _ptr_array(ua), _ptr_array(pa)))
     return ua, pa
+
+
+def write_vtk_fields(config, phys, path, binary=True):
+    """Host-only: the solver's VTK frame of 10 host physical arrays."""
+    pa = [np.ascontiguousarray(a, dtype=np.float64) for a in phys]
+    _check(load().mpd_write_vtk_fields(config.encode(), _ptr_array(pa), str(path).encode(), 1 if binary else 0))
 
 
 class Solver:
@@ -205,6 +212,15 @@ class Solver:
         _check(load().mpd_grid(self._h, r.ctypes.data_as(_DP), rz.ctypes.data_as(_DP), R.ctypes.data_as(_DP),
                                dr.ctypes.data_as(_DP), C.byref(r0)))
         return dict(r=r, r_z=rz, R=R, dr=dr, r_0=r0.value)
+
+    def probe(self, l, m):
+        """Physical values (10, Fields member order) of cell (l, m)."""
+        out = (C.c_double * 10)()
+        _check(load().mpd_probe(self._h, int(l), int(m), out))
+        return np.array(out[:])
+
+    def write_vtk(self, path):
+        _check(load().mpd_write_vtk(self._h, str(path).encode()))
 
     def set_option(self, key, value):
         _check(load().mpd_set_option(self._h, key.encode(), float(value)))

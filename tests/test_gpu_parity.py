@@ -202,3 +202,30 @@ def test_full_size_6400x2560(mpd):
     for k in range(8):
         assert same_bits(ut[k], us[k]), describe_mismatch(ut[k], us[k])
         assert np.isfinite(ut[k][1:-1, 1:-1]).all()
+
+
+@pytest.mark.slow
+def test_shipped_thruster_800x400(mpd):
+    """The reference's shipped case (config.yaml: 800x400 short cathode,
+    3-segment M_LO with the axis_lf axis, expression BCs) for 500 steps of
+    its adaptive time loop: every field, t and dt bit-identical to the
+    oracle."""
+    text = cases.thruster_config(800, 400, max_steps=500)
+    print("shipped thruster 800x400, 500 steps", flush=True)
+    o, ores = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        res = sol.run()
+        assert res["steps"] == ores["steps"] == 500
+        assert res["t"] == ores["t"] and res["dt"] == ores["dt"]
+        _compare_state(sol, o, "thruster_800x400")
+
+
+def test_run_done_flag(mpd):
+    """mpd_run's done reports the time loop's own end (T, convergence or
+    time.max_steps), not the call's step cap."""
+    text = cases.channel_config(20, 9, max_steps=30)
+    with mpd.Solver(text) as sol:
+        r = sol.run(10)
+        assert r["steps"] == 10 and r["done"] == 0
+        r = sol.run()
+        assert r["steps"] == 20 and r["total_steps"] == 30 and r["done"] == 1

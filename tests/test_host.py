@@ -199,3 +199,22 @@ def test_initial_state_full_size_matches_oracle(mpd):
         assert same_bits(p[k], o.phys(k)), O.PHYS_NAMES[k]
     for k in range(8):
         assert same_bits(u[k], o.u0(k)), k
+
+
+SHIPPED = "/root/reference/config.yaml"
+
+
+@pytest.mark.skipif(not os.path.exists(SHIPPED), reason="reference tree not present")
+def test_thruster_config_is_the_shipped_case(mpd):
+    """cases.thruster_config(800, 400) is the reference's shipped config.yaml
+    for the step: same grid and a bit-identical initial state (read from the
+    reference tree where it is present; nothing is copied)."""
+    shipped = open(SHIPPED).read()
+    ours = cases.thruster_config(800, 400)
+    assert mpd.config_check(shipped) == mpd.config_check(ours) == (800, 400)
+    u1, p1 = mpd.initial_state(shipped)
+    u2, p2 = mpd.initial_state(ours)
+    for k in range(8):
+        assert same_bits(u1[k], u2[k]), k
+    for k in range(10):
+        assert same_bits(p1[k], p2[k]), k
