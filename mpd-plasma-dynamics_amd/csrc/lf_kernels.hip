@@ -43,6 +43,21 @@ __device__ __forceinline__ void start_phys(const StepArgs& a, int l, int m, doub
     }
 }
 
+// start_phys and s0_phys (below) split in two, so that the boundary kernel
+// can issue the loads of several cells before it waits for any: where the
+// values of cell (l, m) live (8 conserved values at p + k stride, or on the
+// first step 10 physical values), the loads, and the conversion.
+struct CellSrc {
+    const double* p;
+    int64_t stride;
+    bool phys;
+};
+
+__device__ __forceinline__ CellSrc start_src(const StepArgs& a, int l, int m) {
+    if (a.step1) return {a.phys_in + at(a, l, m), a.fstride, true};
+    return {a.u_in + at(a, l, m), a.fstride, false};
+}
+
 // The physical state every boundary-condition read starts from (the state
 // after UpdateCentralPhysical, src/solver.cpp:155-159): central cells hold
 // f(u_new), every other interior cell the start-of-step physical values.
@@ -58,6 +73,32 @@ __device__ __forceinline__ void s0_phys(const StepArgs& a, int l, int m, double*
         phys10_from_u(u, cell_r(a, l, m), a.gm1, ph);
     } else {
         start_phys(a, l, m, ph);
+    }
+}
+
+__device__ __forceinline__ CellSrc s0_src(const StepArgs& a, int l, int m) {
+    if (m >= 2 && m <= a.Mn - 1) {
+        if (l == 1 || l == a.L) return {a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch + phys_col(m), a.pitch, false};
+        return {a.u_out + at(a, l, m), a.fstride, false};
+    }
+    return start_src(a, l, m);
+}
+
+__device__ __forceinline__ void src_load(const CellSrc& s, double* v) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = s.p[k * s.stride];
+    if (s.phys) {
+        v[8] = s.p[8 * s.stride];
+        v[9] = s.p[9 * s.stride];
+    }
+}
+
+__device__ __forceinline__ void src_phys(const CellSrc& s, const double* v, double r, double gm1, double* ph) {
+    if (s.phys) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) ph[k] = v[k];
+    } else {
+        phys10_from_u(v, r, gm1, ph);
     }
 }
 
@@ -89,6 +130,13 @@ __device__ __forceinline__ void load_rec(const StepArgs& a, int l, int m, Rec& c
     }
 }
 
+// A loop step does nothing once the loop has ended (done) or while the last
+// step waits for its exact wave-speed rescan (fallback): the host runs the
+// rescan, which finishes that step, and launches the remaining steps again.
+__device__ __forceinline__ bool loop_idle(const StepArgs& a) {
+    return (a.st->done | a.st->fallback) != 0;
+}
+
 __device__ __forceinline__ double step_dt(const StepArgs& a, double dt_explicit, int loop) {
     return loop ? a.st->dt_use : dt_explicit;
 }
@@ -98,7 +146,7 @@ __device__ __forceinline__ double step_dt(const StepArgs& a, double dt_explicit,
 // kernel, used for the first step and as a cross-check of the tiled kernel.
 template <bool kStep1, bool kSpeed, class Div>
 __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_explicit, int loop) {
-    if (loop && a.st->done) return;
+    if (loop && loop_idle(a)) return;
     const int m = 2 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int l = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
     const double dt = step_dt(a, dt_explicit, loop);
@@ -318,7 +366,7 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
 template <bool kSpeed, bool kBuf, class Div>
 __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
                                                            int nstrips, int ntiles) {
-    if (loop && a.st->done) return;
+    if (loop && loop_idle(a)) return;
     const int lane = (int)(threadIdx.x & 63);
     // Wave-uniform indices (scalar registers: row addressing and the march
     // loop then run on the scalar unit).  Tiles in launch order, strip
@@ -518,25 +566,29 @@ __device__ __forceinline__ int face_chain(const StepArgs& a, const WorkArea& w, 
     const MpdBcProgram* bc = a.bc;
     const bool is_l = (f == MPD_L_LO || f == MPD_L_HI);
     const int i = is_l ? m : l;
+    // The geometry symbols, read before the segment loop so that their loads
+    // overlap the reads of the program.
+    int ln, mn;
+    face_nb(a, f, l, m, ln, mn);
+    const double g_dr = a.dr[l], g_r = cell_r(a, l, m), g_rz = cell_rz(a, l, m);
+    const double g_rnb = cell_r(a, ln, mn), g_rznb = cell_rz(a, ln, mn);
     int last = -1;
     for (int s = 0; s < seg_end; ++s) {
         const MpdSeg& g = bc->seg[f][s];
         if (!seg_covers(g, i)) continue;
         last = s;
         if (g.has_expr) {
-            int ln, mn;
-            face_nb(a, f, l, m, ln, mn);
             w.sym(MPD_BS_GAMMA) = a.gamma;
             w.sym(MPD_BS_BETA) = a.beta;
             w.sym(MPD_BS_HZ0) = a.H_z0;
             w.sym(MPD_BS_R0) = a.r_0;
             w.sym(MPD_BS_DZ) = a.dz;
-            w.sym(MPD_BS_DR) = a.dr[l];
-            w.sym(MPD_BS_R) = cell_r(a, l, m);
-            w.sym(MPD_BS_RZ) = cell_rz(a, l, m);
+            w.sym(MPD_BS_DR) = g_dr;
+            w.sym(MPD_BS_R) = g_r;
+            w.sym(MPD_BS_RZ) = g_rz;
             w.sym(MPD_BS_Z) = (double)(l - 1) * a.dz;
-            w.sym(MPD_BS_RNB) = cell_r(a, ln, mn);
-            w.sym(MPD_BS_RZNB) = cell_rz(a, ln, mn);
+            w.sym(MPD_BS_RNB) = g_rnb;
+            w.sym(MPD_BS_RZNB) = g_rznb;
 #pragma unroll
             for (int k = 0; k < 8; ++k) w.sym(MPD_BS_RHO_NB + k) = w.nbr(nb_slot(k));
         }
@@ -572,28 +624,44 @@ __device__ __forceinline__ void rebuild_u(const double* ph, double r, double* u)
 // segment s (serial order, l ascending): the M_LO segments before s, plus
 // segment s itself when `with_s` (the cell was already visited in s).  Cell
 // (1,1) belongs to L_LO, which ran before; cell (L,1) to L_HI, which runs
-// after.
-__device__ __forceinline__ void mlo_state(const StepArgs& a, const WorkArea& w, int l, int s, bool with_s,
-                                          double* ph) {
-    s0_phys(a, l, 1, ph);
-    int cf = -1, ce = 0, nl = l, nm = 2;
+// after.  In two parts: mlo_plan issues the loads, mlo_finish runs the chain.
+struct MloPlan {
+    CellSrc s, n;
+    int l, cf, ce, nl, nm;
+    double v[10], vn[10];
+};
+
+__device__ __forceinline__ void mlo_plan(const StepArgs& a, int l, int s, bool with_s, MloPlan& p) {
+    p.l = l;
+    p.cf = -1;
+    p.ce = 0;
+    p.nl = l;
+    p.nm = 2;
     if (l == 1) {
         if (face_covers(a.bc, MPD_L_LO, 1)) {
-            cf = MPD_L_LO;
-            ce = a.bc->nseg[MPD_L_LO];
-            nl = 2;
-            nm = 1;
+            p.cf = MPD_L_LO;
+            p.ce = a.bc->nseg[MPD_L_LO];
+            p.nl = 2;
+            p.nm = 1;
         }
     } else if (l != a.L) {
-        cf = MPD_M_LO;
-        ce = with_s ? s + 1 : s;
+        p.cf = MPD_M_LO;
+        p.ce = with_s ? s + 1 : s;
     }
-    if (cf >= 0) {
+    p.s = s0_src(a, l, 1);
+    p.n = s0_src(a, p.nl, p.nm);
+    src_load(p.s, p.v);
+    if (p.cf >= 0) src_load(p.n, p.vn);
+}
+
+__device__ __forceinline__ void mlo_finish(const StepArgs& a, const WorkArea& w, const MloPlan& p, double* ph) {
+    src_phys(p.s, p.v, cell_r(a, p.l, 1), a.gm1, ph);
+    if (p.cf >= 0) {
         double nb[10];
-        s0_phys(a, nl, nm, nb);
+        src_phys(p.n, p.vn, cell_r(a, p.nl, p.nm), a.gm1, nb);
         w.put(WA_CUR, ph);
         w.put(WA_NBR, nb);
-        face_chain(a, w, cf, l, 1, ce);
+        face_chain(a, w, p.cf, p.l, 1, p.ce);
         w.get(WA_CUR, ph);
     }
 }
@@ -651,14 +719,10 @@ __device__ __forceinline__ void axis_overrides(const StepArgs& a, const MpdSeg& 
 // face in L_LO, M_HI, M_LO, L_HI order that covers it).
 __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, int f, int l, int m, double dt,
                                          double* u) {
-    double st[10], nb[10];
-    // Own state before the face: only AxisLF conditions (M_LO only) read it.
-    s0_phys(a, l, m, st);
-    // Neighbour state before the face (src/face_bc.cpp order: L_LO, M_HI,
-    // M_LO, L_HI; only corners see an earlier face).
+    // Neighbour of the face (src/face_bc.cpp order: L_LO, M_HI, M_LO, L_HI;
+    // only corners see an earlier face at their neighbour).
     int ln, mn;
     face_nb(a, f, l, m, ln, mn);
-    s0_phys(a, ln, mn, nb);
     int pf = -1, pl = 0, pm = 0, ql = 0, qm = 0;  // earlier face at the neighbour, its cell and neighbour
     if (f == MPD_M_HI && ln == 1 && face_covers(a.bc, MPD_L_LO, mn)) {
         pf = MPD_L_LO; pl = 1; pm = mn; ql = 2; qm = mn;
@@ -667,9 +731,20 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
     } else if (f == MPD_L_HI && mn == a.Mn && face_covers(a.bc, MPD_M_HI, ln)) {
         pf = MPD_M_HI; pl = ln; pm = a.Mn; ql = ln; qm = a.Mn - 1;
     }
+    // Every load of the chain first: own state (only AxisLF conditions read
+    // it), neighbour state, and the earlier face's neighbour.
+    const CellSrc so = s0_src(a, l, m), sn = s0_src(a, ln, mn);
+    const CellSrc sq = s0_src(a, pf >= 0 ? ql : ln, pf >= 0 ? qm : mn);
+    double vo[10], vn[10], vq[10];
+    src_load(so, vo);
+    src_load(sn, vn);
+    if (pf >= 0) src_load(sq, vq);
+    double st[10], nb[10];
+    src_phys(so, vo, cell_r(a, l, m), a.gm1, st);
+    src_phys(sn, vn, cell_r(a, ln, mn), a.gm1, nb);
     if (pf >= 0) {
         double nb2[10];
-        s0_phys(a, ql, qm, nb2);
+        src_phys(sq, vq, cell_r(a, ql, qm), a.gm1, nb2);
         w.put(WA_CUR, nb);
         w.put(WA_NBR, nb2);
         face_chain(a, w, pf, pl, pm, a.bc->nseg[pf]);
@@ -683,12 +758,17 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
     rebuild_u(st, r, u);
     if (f == MPD_M_LO && last >= 0 && a.bc->seg[f][last].has_axis) {
         const MpdSeg& g = a.bc->seg[f][last];
+        MloPlan plo, phi;
+        mlo_plan(a, l - 1, last, seg_covers(g, l - 1), plo);
+        mlo_plan(a, l + 1, last, false, phi);
+        // The start-of-step values at (l-1,1), (l,1), (l+1,1) are the
+        // values loaded for the plans' own states and for this cell's.
         double lo[10], hi[10], sp_lo[10], sp_c[10], sp_hi[10];
-        mlo_state(a, w, l - 1, last, seg_covers(g, l - 1), lo);
-        mlo_state(a, w, l + 1, last, false, hi);
-        start_phys(a, l - 1, 1, sp_lo);
-        start_phys(a, l, 1, sp_c);
-        start_phys(a, l + 1, 1, sp_hi);
+        src_phys(plo.s, plo.v, cell_r(a, l - 1, 1), a.gm1, sp_lo);
+        src_phys(phi.s, phi.v, cell_r(a, l + 1, 1), a.gm1, sp_hi);
+        src_phys(so, vo, r, a.gm1, sp_c);
+        mlo_finish(a, w, plo, lo);
+        mlo_finish(a, w, phi, hi);
         axis_overrides(a, g, l, dt, st, lo, hi, nb, sp_lo, sp_c, sp_hi, u);
     }
 }
@@ -719,26 +799,64 @@ __device__ void finalize_step(const StepArgs& a) {
     st->ticket = 0;
 }
 
+// Copies the parts of the BC program in use into `prog` (LDS, same layout):
+// the header, the used segments of each face and the expression code.  The
+// header is read first; then each thread issues all its loads before its LDS
+// stores, so the copy costs two global round trips (a plain strided copy
+// loop waits for each load in turn: 10 us measured at 64 threads).
+__device__ __forceinline__ void stage_program(const MpdBcProgram* bc, uint64_t* prog) {
+    constexpr int kHead = (int)(offsetof(MpdBcProgram, seg) / 8);
+    constexpr int kSegW = (int)(sizeof(MpdSeg) / 8);
+    constexpr int kFaceW = kSegW * MPD_MAX_SEGS_PER_FACE;
+    constexpr int kCode = (int)(offsetof(MpdBcProgram, code) / 8);
+    static_assert(offsetof(MpdBcProgram, seg) % 8 == 0 && sizeof(MpdSeg) % 8 == 0 && sizeof(MpdInstr) == 16, "");
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(bc);
+    const int w0 = kSegW * bc->nseg[0];
+    const int w1 = w0 + kSegW * bc->nseg[1];
+    const int w2 = w1 + kSegW * bc->nseg[2];
+    const int w3 = w2 + kSegW * bc->nseg[3];
+    const int total = kHead + w3 + 2 * bc->ncode;
+    constexpr int kPer = 8;
+    for (int base = 0; base < total; base += kPer * BC_THREADS) {
+        uint64_t v[kPer];
+        int d[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            int i = base + j * BC_THREADS + (int)threadIdx.x;
+            d[j] = -1;
+            if (i < total) {
+                int o;
+                if (i < kHead) o = i;
+                else if ((i -= kHead) < w0) o = kHead + i;
+                else if (i < w1) o = kHead + kFaceW + (i - w0);
+                else if (i < w2) o = kHead + 2 * kFaceW + (i - w1);
+                else if (i < w3) o = kHead + 3 * kFaceW + (i - w2);
+                else o = kCode + (i - w3);
+                d[j] = o;
+                v[j] = src[o];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            if (d[j] >= 0) prog[d[j]] = v[j];
+    }
+    __syncthreads();
+}
+
 // One thread per boundary cell: row 1, row L, column 1 (l = 2..L-1), column
 // Mn (l = 2..L-1).  Writes each cell's final conserved values, adds its wave
 // speed to the running max, and the last block finishes the step.
 __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double dt_explicit, int loop, int speed) {
-    if (loop && a_in.st->done) return;
+    if (loop && loop_idle(a_in)) return;
     // The BC program (segments and the expression code in use) is read by
     // every thread many times over in dependent chains: copy it to LDS once.
     __shared__ uint64_t prog[sizeof(MpdBcProgram) / 8];
     __shared__ double area[WA_N * BC_THREADS];
-    {
-        const uint64_t* src = reinterpret_cast<const uint64_t*>(a_in.bc);
-        const int ncode = a_in.bc->ncode;
-        const int nw = (int)((offsetof(MpdBcProgram, code) + (size_t)ncode * sizeof(MpdInstr)) / 8);
-        for (int i = (int)threadIdx.x; i < nw; i += (int)blockDim.x) prog[i] = src[i];
-        __syncthreads();
-    }
+    const double dt = step_dt(a_in, dt_explicit, loop);
+    stage_program(a_in.bc, prog);
     StepArgs a = a_in;
     a.bc = reinterpret_cast<const MpdBcProgram*>(prog);
     const WorkArea w{area, (int)threadIdx.x};
-    const double dt = step_dt(a, dt_explicit, loop);
     const int L = a.L, Mn = a.Mn;
     const int nring = 2 * Mn + 2 * (L - 2);
     const int idx = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -855,7 +973,7 @@ __global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, dou
                                                    int check_frequency, double threshold, int save) {
     if (loop) {
         const int step = a.st->step;
-        if (step % check_frequency != 0 || step == a.st->checked_step) return;
+        if (a.st->fallback || step % check_frequency != 0 || step == a.st->checked_step) return;
     }
     const int L = a.L, Mn = a.Mn;
     const int64_t n = (int64_t)L * Mn;

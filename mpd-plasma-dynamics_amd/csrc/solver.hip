@@ -469,13 +469,32 @@ struct mpd_solver {
         return k + (k & 1);
     }
 
-    void EnqueueLoopStep(int in, bool conv) {
+    // One step of the time loop.  `rescan`: also the exact wave-speed rescan
+    // kernel, which finishes the step when its candidate max could not be
+    // proven (rare; it exits at once otherwise).  The graphs leave it out: a
+    // step that needs it idles the rest of the graph and the host finishes it
+    // (ResolveRescan) before it launches the remaining steps.
+    void EnqueueLoopStep(int in, bool conv, bool rescan) {
         const StepArgs a = Args(in);
         HIPCHK(mpd::launch_stencil(a, scfg, 0.0, 1, 1, stream));
         HIPCHK(mpd::launch_bc(a, 0.0, 1, 1, stream));
-        HIPCHK(mpd::launch_speed_fallback(a, stream));
+        if (rescan) HIPCHK(mpd::launch_speed_fallback(a, stream));
         if (conv) {
             StepArgs r = Args(1 - in);
+            r.step1 = 0;
+            HIPCHK(mpd::launch_residual(r, prev, partial, 1, cfg.check_frequency, cfg.convergence_threshold, 1,
+                                        stream));
+        }
+    }
+
+    // Finishes the step that waits for its exact wave-speed rescan (loop
+    // state `fallback`; cur is that step's input): the rescan with the step's
+    // dt bookkeeping, then the step's convergence check.
+    void ResolveRescan(bool conv) {
+        const StepArgs a = Args(cur);
+        HIPCHK(mpd::launch_speed_fallback(a, stream));
+        if (conv) {
+            StepArgs r = Args(1 - cur);
             r.step1 = 0;
             HIPCHK(mpd::launch_residual(r, prev, partial, 1, cfg.check_frequency, cfg.convergence_threshold, 1,
                                         stream));
@@ -491,7 +510,7 @@ struct mpd_solver {
         for (int p = 0; p < 2; ++p) {
             hipGraph_t gr = nullptr;
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < K; ++i) EnqueueLoopStep((p + i) & 1, conv);
+            for (int i = 0; i < K; ++i) EnqueueLoopStep((p + i) & 1, conv, false);
             HIPCHK(hipStreamEndCapture(stream, &gr));
             HIPCHK(hipGraphInstantiate(&graph[p], gr, nullptr, nullptr, 0));
             HIPCHK(hipGraphDestroy(gr));
@@ -549,7 +568,7 @@ struct mpd_solver {
         if (conv && !prev) throw std::runtime_error("convergence snapshot missing");
         bool done = SeedLoop(max_steps);
         if (!done && step1) {
-            EnqueueLoopStep(cur, conv);
+            EnqueueLoopStep(cur, conv, true);
             const MpdLoopState h = ReadLoop();
             Absorb(h);
             done = h.done != 0;
@@ -558,8 +577,13 @@ struct mpd_solver {
         while (!done) {
             if (!graph[0] || graph_K != K) CaptureGraphs(K, conv);
             HIPCHK(hipGraphLaunch(graph[cur], stream));
-            const MpdLoopState h = ReadLoop();
+            MpdLoopState h = ReadLoop();
             Absorb(h);
+            if (!h.done && h.fallback) {
+                ResolveRescan(conv);
+                h = ReadLoop();
+                Absorb(h);
+            }
             done = h.done != 0;
         }
         if (out) {
@@ -583,21 +607,45 @@ struct mpd_solver {
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        // Whole loop, graphs back to back.
+        // Whole loop, graphs back to back, then the steps short of a whole
+        // graph one by one (each with its rescan kernel), in a segment of
+        // their own: a graph step that waits for its exact wave-speed rescan
+        // idles the rest of the segment, the host runs the rescan and
+        // launches the remaining steps (all of it timed).
         if (SeedLoop(steps)) throw std::runtime_error("time loop already finished (t >= T or converged)");
-        const int ngraph = steps / K, rem = steps % K;
-        HIPCHK(hipEventRecord(e0, stream));
-        int c = cur;
-        for (int i = 0; i < ngraph; ++i) HIPCHK(hipGraphLaunch(graph[c], stream));
-        for (int i = 0; i < rem; ++i) {
-            EnqueueLoopStep(c, conv);
-            c ^= 1;
+        const int target = step + steps;
+        double ms = 0.0;
+        for (;;) {
+            const int todo = target - step;
+            HIPCHK(hipEventRecord(e0, stream));
+            if (todo >= K) {
+                for (int i = 0; i < todo / K; ++i) HIPCHK(hipGraphLaunch(graph[cur], stream));
+            } else {
+                for (int i = 0; i < todo; ++i) EnqueueLoopStep((cur + i) & 1, conv, true);
+            }
+            HIPCHK(hipEventRecord(e1, stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float part = 0.f;
+            HIPCHK(hipEventElapsedTime(&part, e0, e1));
+            ms += part;
+            const int before = step;
+            MpdLoopState h = ReadLoop();
+            Absorb(h);
+            if (h.done) break;
+            if (!h.fallback) {
+                if (step == before) throw std::runtime_error("time loop made no progress");
+                continue;
+            }
+            HIPCHK(hipEventRecord(e0, stream));
+            ResolveRescan(conv);
+            HIPCHK(hipEventRecord(e1, stream));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&part, e0, e1));
+            ms += part;
+            h = ReadLoop();
+            Absorb(h);
+            if (h.done) break;
         }
-        HIPCHK(hipEventRecord(e1, stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        Absorb(ReadLoop());
         if (total_ms) *total_ms = ms;
         // Per-kernel averages: each launch bracketed by events.
         if (stencil_ms || bc_ms) {
@@ -612,8 +660,8 @@ struct mpd_solver {
                 HIPCHK(mpd::launch_stencil(a, scfg, 0.0, 1, 1, stream));
                 HIPCHK(hipEventRecord(ev[3 * i + 1], stream));
                 HIPCHK(mpd::launch_bc(a, 0.0, 1, 1, stream));
-                HIPCHK(mpd::launch_speed_fallback(a, stream));
                 HIPCHK(hipEventRecord(ev[3 * i + 2], stream));
+                HIPCHK(mpd::launch_speed_fallback(a, stream));
                 if (conv)
                     HIPCHK(mpd::launch_residual(Args(1 - in), prev, partial, 1, cfg.check_frequency,
                                                 cfg.convergence_threshold, 1, stream));

@@ -62,6 +62,69 @@ def test_tiled_odd_tile_rows(mpd):
             _compare_state(sol, o, f"tile_rows={h}")
 
 
+def _spike_config(L=60, M=24, steps=80):
+    """Straight channel with a one-cell v_z spike at (z, r) = (0.5, 0.5): the
+    LF averaging flattens it, so the max wave speed falls by far more than
+    0.1 % per step for many steps."""
+    text = cases.channel_config(L, M, max_steps=steps)
+    spike = '"0.1 + 40 * max(0, 1 - 300 * abs(z - 0.5)) * max(0, 1 - 250 * abs(r - 0.5))"'
+    assert text.count("  v_z: 0.1\n") == 1
+    return text.replace("  v_z: 0.1\n", f"  v_z: {spike}\n")
+
+
+def _oracle_speed_drops(text, steps):
+    """Steps at which the oracle's max wave speed falls below 0.999 x the
+    previous step's: each needs the exact rescan of the whole state."""
+    case = cases.oracle_case(text)
+    o = O.Oracle(case)
+    t, dt = 0.0, case["dt"]
+    prev = None
+    drops = []
+    for k in range(1, steps + 1):
+        res = o.run(1, t=t, dt=dt)
+        t, dt = res["t"], res["dt"]
+        s = o.max_wave_speed()
+        if prev is not None and s < 0.999 * prev:
+            drops.append(k)
+        prev = s
+    return o, t, dt, drops
+
+
+@pytest.mark.parametrize("per_graph", [0, 6])
+def test_wave_speed_rescan_in_graphs(mpd, per_graph):
+    """Steps whose candidate max cannot be proven idle the rest of their graph;
+    the host runs the exact rescan, which finishes the step, and launches the
+    remaining steps.  The result is the oracle's, bit for bit."""
+    steps = 80
+    text = _spike_config(steps=steps)
+    o, t, dt, drops = _oracle_speed_drops(text, steps)
+    assert len(drops) >= 10, drops
+    print(f"rescan steps: {len(drops)} of {steps}, steps_per_graph={per_graph}", flush=True)
+    with mpd.Solver(text) as sol:
+        if per_graph:
+            sol.set_option("steps_per_graph", per_graph)
+        res = sol.run()
+        assert res["steps"] == steps and res["t"] == t and res["dt"] == dt
+        _compare_state(sol, o, "spike")
+
+
+@pytest.mark.parametrize("per_graph", [0, 4])
+def test_wave_speed_rescan_in_timed_loop(mpd, per_graph):
+    """The benchmark loop (mpd_time_steps: warm-up, timed graphs, per-kernel
+    pass) with rescans; afterwards the state is the oracle's after all its
+    steps."""
+    warm, steps = 5, 10
+    text = _spike_config(steps=0)
+    o, t, dt, drops = _oracle_speed_drops(text, warm + 2 * steps)
+    assert len(drops) >= 10, drops
+    with mpd.Solver(text) as sol:
+        if per_graph:
+            sol.set_option("steps_per_graph", per_graph)
+        tm = sol.time_steps(steps, warm)
+        assert tm["total_ms"] > 0
+        _compare_state(sol, o, "spike timed")
+
+
 def test_advance_and_compute_dt_match_oracle(mpd):
     """Solver::Advance(dt) + Diagnostics::ComputeDt driven from the host, as
     src/main.cpp:225-231 drives them."""
