@@ -133,6 +133,45 @@ int mpd_set_option(mpd_solver* s, const char* key, double value);
 int mpd_time_steps(mpd_solver* s, int steps, int warmup, double* total_ms, double* stencil_ms,
                    double* bc_ms);
 
+/* ------------------------------------------------------------------------
+ * Batched parameter sweeps.  No counterpart in the reference, which runs one
+ * case per process (src/main.cpp:22-317); a sweep over inflow parameters,
+ * wall shapes and grid sizes is N such runs.  A batch holds N independent
+ * instances, each built from its own config text exactly as
+ * mpd_create_from_string builds one, and advances them together: one stencil
+ * launch, one boundary launch and one convergence launch per step for all
+ * instances (a work list of (instance, tile) entries), K steps per HIP graph.
+ * Each instance's results are bit-identical to running it alone.
+ */
+typedef struct mpd_batch mpd_batch;
+
+/* Build n instances from n config texts (host setup in parallel threads). */
+int mpd_batch_create(const char* const* yaml_texts, int n, mpd_batch** out);
+void mpd_batch_destroy(mpd_batch* b);
+int mpd_batch_size(const mpd_batch* b, int* n);
+
+/* Borrowed handle of instance i: every single-solver call above works on it
+ * (upload a warm start, download, integrals, VTK ...); it lives as long as
+ * the batch and mpd_destroy ignores it. */
+int mpd_batch_instance(mpd_batch* b, int i, mpd_solver** out);
+
+/* Options: "steps_per_graph" (0: the smallest convergence check_frequency of
+ * the running instances), "grid" (stencil workgroups; 0: one per work entry,
+ * fewer: a persistent grid striding over the list), "tile_rows" (all
+ * instances). */
+int mpd_batch_set_option(mpd_batch* b, const char* key, double value);
+
+/* Autotuner: times the batched stencil launch on the current states (reps
+ * launches per trial) and picks the tile height of each distinct grid size
+ * and the stencil grid by coordinate search; writes a text report. */
+int mpd_batch_autotune(mpd_batch* b, int reps, char* report, int report_len);
+int mpd_batch_tile_rows(const mpd_batch* b, int i, int* tile_rows);
+
+/* The time loop of every instance (as mpd_run), at most max_steps more steps
+ * each (<= 0: no cap besides time.max_steps), until every instance is done;
+ * results: n entries, may be NULL. */
+int mpd_batch_run(mpd_batch* b, int max_steps, mpd_run_result* results);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,28 @@ struct StepArgs {
     const MpdBcProgram* bc;
 };
 
+// One workgroup's entry in a batched launch's work list: its instance and,
+// for the stencil, the first of its four tiles, the instance's tile count,
+// tile rows and strip count; for the boundary and residual kernels, the
+// block index within the instance and the instance's block count.
+struct BatchBlock {
+    int inst;
+    int first;
+    int count;
+    int H;
+    int nstrips;
+};
+
+// Per-instance convergence data of a batched loop (check_frequency 0: no
+// convergence check for that instance).
+struct BatchAux {
+    double* prev;
+    double* partial;
+    double threshold;
+    int check_frequency;
+    int pad;
+};
+
 // Physical fields in the member order of include/fields.hpp:36-41.
 enum { PH_RHO = 0, PH_VZ, PH_VR, PH_VPHI, PH_E, PH_P, PH_PT, PH_HZ, PH_HR, PH_HPHI };
 

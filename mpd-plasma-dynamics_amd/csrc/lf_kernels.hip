@@ -16,6 +16,8 @@
 //                     radial-symmetry metric, physical-field download.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lf_device.h"
 #include "lf_launch.h"
 
@@ -363,19 +365,12 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
 // a row covers the whole slot: lane i writes slot (i + 63) mod 64, its own
 // cell for lanes 1..62 and an unused double for lanes 0 and 63; lanes of the
 // boundary and ghost columns 0, 1, M_max+1, M_max+2 do not store.
+// One wavefront's tile (strip, chunk of H rows) of the march; `tile` is
+// wave-uniform.
 template <bool kSpeed, bool kBuf, class Div>
-__global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
-                                                           int nstrips, int ntiles) {
-    if (loop && loop_idle(a)) return;
+__device__ __forceinline__ void stencil_tile(const StepArgs& a, double dt_explicit, int loop, int H, int nstrips,
+                                             int tile) {
     const int lane = (int)(threadIdx.x & 63);
-    // Wave-uniform indices (scalar registers: row addressing and the march
-    // loop then run on the scalar unit).  Tiles in launch order, strip
-    // fastest: the four waves of a workgroup take four adjacent strips of one
-    // band of rows, and the whole GPU works on a narrow band at a time (an
-    // order giving each XCD its own band of rows measured 5% slower).
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int tile = (int)blockIdx.x * 4 + wave;
-    if (tile >= ntiles) return;
     const int strip = tile % nstrips, chunk = tile / nstrips;
     TileCtx x;
     x.m = kStripCols * strip - 1 + lane;
@@ -415,6 +410,46 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt
     if (kSpeed) {
         key = wave_max_u64(key);
         if (lane == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
+    }
+}
+
+template <bool kSpeed, bool kBuf, class Div>
+__global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
+                                                           int nstrips, int ntiles) {
+    if (loop && loop_idle(a)) return;
+    // Wave-uniform indices (scalar registers: row addressing and the march
+    // loop then run on the scalar unit).  Tiles in launch order, strip
+    // fastest: the four waves of a workgroup take four adjacent strips of one
+    // band of rows, and the whole GPU works on a narrow band at a time (an
+    // order giving each XCD its own band of rows measured 5% slower).
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tile = (int)blockIdx.x * 4 + wave;
+    if (tile >= ntiles) return;
+    stencil_tile<kSpeed, kBuf, Div>(a, dt_explicit, loop, H, nstrips, tile);
+}
+
+// The tiled stencil over many instances of uneven size in one launch.  The
+// work list holds one entry per workgroup: its instance and the first of its
+// four tiles in that instance (an instance's tiles are never shared with
+// another instance's in one workgroup).  Workgroups stride over the list, so
+// the grid may be smaller than the list (a persistent grid sized to the
+// GPU).  Instance i's step reads args[2 i + (icur[i] ^ flip)]: its two
+// ping-pong orientations, icur[i] being the state buffer it starts the graph
+// from and flip the parity of the step within the graph.
+template <bool kSpeed, bool kBuf, class Div>
+__global__ void __launch_bounds__(256, 2) lf_stencil_tiled_batch(const StepArgs* __restrict__ args,
+                                                                 const int* __restrict__ icur,
+                                                                 const BatchBlock* __restrict__ work, int nwork,
+                                                                 int flip, double dt_explicit, int loop) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int b = (int)blockIdx.x; b < nwork; b += (int)gridDim.x) {
+        const BatchBlock w = work[b];
+        const int inst = __builtin_amdgcn_readfirstlane(w.inst);
+        const StepArgs a = args[2 * inst + (icur[inst] ^ flip)];
+        if (loop && loop_idle(a)) continue;
+        const int tile = __builtin_amdgcn_readfirstlane(w.first) + wave;
+        if (tile >= w.count) continue;
+        stencil_tile<kSpeed, kBuf, Div>(a, dt_explicit, loop, w.H, w.nstrips, tile);
     }
 }
 
@@ -846,12 +881,10 @@ __device__ __forceinline__ void stage_program(const MpdBcProgram* bc, uint64_t* 
 // One thread per boundary cell: row 1, row L, column 1 (l = 2..L-1), column
 // Mn (l = 2..L-1).  Writes each cell's final conserved values, adds its wave
 // speed to the running max, and the last block finishes the step.
-__global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double dt_explicit, int loop, int speed) {
-    if (loop && loop_idle(a_in)) return;
-    // The BC program (segments and the expression code in use) is read by
-    // every thread many times over in dependent chains: copy it to LDS once.
-    __shared__ uint64_t prog[sizeof(MpdBcProgram) / 8];
-    __shared__ double area[WA_N * BC_THREADS];
+// Block `bid` of the `nblk` blocks covering one instance's ring; prog and
+// area are the block's LDS.
+__device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt_explicit, int loop, int speed, int bid,
+                                              int nblk, uint64_t* prog, double* area) {
     const double dt = step_dt(a_in, dt_explicit, loop);
     stage_program(a_in.bc, prog);
     StepArgs a = a_in;
@@ -859,7 +892,7 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
     const WorkArea w{area, (int)threadIdx.x};
     const int L = a.L, Mn = a.Mn;
     const int nring = 2 * Mn + 2 * (L - 2);
-    const int idx = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int idx = bid * BC_THREADS + (int)threadIdx.x;
     uint64_t key = 0;
     if (idx < nring) {
         int l, m;
@@ -911,7 +944,7 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
     if (threadIdx.x == 0) {
         __threadfence();
         const unsigned t = atomicAdd(&a.st->ticket, 1u);
-        last = (t == gridDim.x - 1);
+        last = (t == (unsigned)nblk - 1);
     }
     __syncthreads();
     if (last && threadIdx.x == 0) {
@@ -928,6 +961,30 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
             finalize_step(a);
         }
     }
+}
+
+__global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double dt_explicit, int loop, int speed) {
+    if (loop && loop_idle(a_in)) return;
+    // The BC program (segments and the expression code in use) is read by
+    // every thread many times over in dependent chains: copy it to LDS once.
+    __shared__ uint64_t prog[sizeof(MpdBcProgram) / 8];
+    __shared__ double area[WA_N * BC_THREADS];
+    bc_ring_block(a_in, dt_explicit, loop, speed, (int)blockIdx.x, (int)gridDim.x, prog, area);
+}
+
+// The boundary kernel over many instances: work entry b = (instance, block
+// index within the instance's ring, the instance's block count); the last
+// block of each instance finishes that instance's step.
+__global__ void __launch_bounds__(BC_THREADS) lf_bc_ring_batch(const StepArgs* __restrict__ args,
+                                                               const int* __restrict__ icur,
+                                                               const BatchBlock* __restrict__ work, int flip,
+                                                               int speed) {
+    const BatchBlock w = work[blockIdx.x];
+    const StepArgs a = args[2 * w.inst + (icur[w.inst] ^ flip)];
+    if (loop_idle(a)) return;
+    __shared__ uint64_t prog[sizeof(MpdBcProgram) / 8];
+    __shared__ double area[WA_N * BC_THREADS];
+    bc_ring_block(a, 0.0, 1, speed, w.first, w.count, prog, area);
 }
 
 // Exact max wave speed over the whole new state, run when the candidate max
@@ -969,16 +1026,12 @@ __global__ void __launch_bounds__(256) lf_speed_fallback(StepArgs a) {
 // In the loop (`loop`), it runs only on a step that is a multiple of
 // check_frequency and was not checked yet, and sets converged/done
 // (src/main.cpp:233-247).
-__global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, double* partial, int loop,
-                                                   int check_frequency, double threshold, int save) {
-    if (loop) {
-        const int step = a.st->step;
-        if (a.st->fallback || step % check_frequency != 0 || step == a.st->checked_step) return;
-    }
+__device__ __forceinline__ void residual_block(const StepArgs& a, double* prev, double* partial, int loop,
+                                               int check_frequency, double threshold, int save, int bid, int nblk) {
     const int L = a.L, Mn = a.Mn;
     const int64_t n = (int64_t)L * Mn;
     double sd = 0.0, sc = 0.0;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t c = (int64_t)bid * blockDim.x + threadIdx.x; c < n; c += (int64_t)nblk * blockDim.x) {
         const int l = 1 + (int)(c / Mn), m = 1 + (int)(c % Mn);
         const int64_t i = at(a, l, m);
         double ph[10];
@@ -1008,17 +1061,17 @@ __global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, dou
     }
     __shared__ bool last;
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = sh_d[0];
-        partial[2 * blockIdx.x + 1] = sh_c[0];
+        partial[2 * bid] = sh_d[0];
+        partial[2 * bid + 1] = sh_c[0];
         __threadfence();
         const unsigned t = atomicAdd(&a.st->ticket2, 1u);
-        last = (t == gridDim.x - 1);
+        last = (t == (unsigned)nblk - 1);
     }
     __syncthreads();
     if (last && threadIdx.x == 0) {
         __threadfence();
         double gd = 0.0, gc = 0.0;
-        for (unsigned b = 0; b < gridDim.x; ++b) {
+        for (int b = 0; b < nblk; ++b) {
             gd += __hip_atomic_load(&partial[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             gc += __hip_atomic_load(&partial[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -1034,6 +1087,34 @@ __global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, dou
         }
         a.st->ticket2 = 0;
     }
+}
+
+// A loop check runs only on a step that is a multiple of check_frequency
+// and was not checked yet (src/main.cpp:233-247).
+__device__ __forceinline__ bool residual_due(const StepArgs& a, int check_frequency) {
+    const int step = a.st->step;
+    return !(a.st->fallback || step % check_frequency != 0 || step == a.st->checked_step);
+}
+
+__global__ void __launch_bounds__(256) lf_residual(StepArgs a, double* prev, double* partial, int loop,
+                                                   int check_frequency, double threshold, int save) {
+    if (loop && !residual_due(a, check_frequency)) return;
+    residual_block(a, prev, partial, loop, check_frequency, threshold, save, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// The loop's SolutionChange + SavePrev over many instances: work entry b =
+// (instance, block index, the instance's block count).  Instance i reads the
+// state its step wrote, args[2 i + (icur[i] ^ flip ^ 1)].
+__global__ void __launch_bounds__(256) lf_residual_batch(const StepArgs* __restrict__ args,
+                                                         const int* __restrict__ icur,
+                                                         const BatchAux* __restrict__ aux,
+                                                         const BatchBlock* __restrict__ work, int flip) {
+    const BatchBlock w = work[blockIdx.x];
+    StepArgs a = args[2 * w.inst + (icur[w.inst] ^ flip ^ 1)];
+    const BatchAux x = aux[w.inst];
+    if (x.check_frequency <= 0 || !residual_due(a, x.check_frequency)) return;
+    a.step1 = 0;
+    residual_block(a, x.prev, x.partial, 1, x.check_frequency, x.threshold, 1, w.first, w.count);
 }
 
 // Max wave speed of the state u_in over the interior (or of the IC physical
@@ -1238,6 +1319,34 @@ hipError_t launch_bc(const StepArgs& a, double dt, int loop, int speed, hipStrea
     const int nring = 2 * a.Mn + 2 * (a.L - 2);
     hipLaunchKernelGGL(lf_bc_ring, dim3((unsigned)((nring + BC_THREADS - 1) / BC_THREADS)), dim3(BC_THREADS), 0, s, a,
                        dt, loop, speed);
+    return hipGetLastError();
+}
+
+hipError_t launch_stencil_batch(const StepArgs* args, const int* icur, const BatchBlock* work, int nwork, int grid,
+                                int flip, bool fast_div, bool buf, double dt, int loop, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    const dim3 g((unsigned)std::max(1, std::min(grid > 0 ? grid : nwork, nwork)));
+    if (buf) {
+        if (fast_div) hipLaunchKernelGGL((lf_stencil_tiled_batch<true, true, DivConst>), g, dim3(256), 0, s, args, icur, work, nwork, flip, dt, loop);
+        else hipLaunchKernelGGL((lf_stencil_tiled_batch<true, true, DivExact>), g, dim3(256), 0, s, args, icur, work, nwork, flip, dt, loop);
+    } else {
+        if (fast_div) hipLaunchKernelGGL((lf_stencil_tiled_batch<true, false, DivConst>), g, dim3(256), 0, s, args, icur, work, nwork, flip, dt, loop);
+        else hipLaunchKernelGGL((lf_stencil_tiled_batch<true, false, DivExact>), g, dim3(256), 0, s, args, icur, work, nwork, flip, dt, loop);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_batch(const StepArgs* args, const int* icur, const BatchBlock* work, int nwork, int flip,
+                           hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(lf_bc_ring_batch, dim3((unsigned)nwork), dim3(BC_THREADS), 0, s, args, icur, work, flip, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_batch(const StepArgs* args, const int* icur, const BatchAux* aux, const BatchBlock* work,
+                                 int nwork, int flip, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(lf_residual_batch, dim3((unsigned)nwork), dim3(256), 0, s, args, icur, aux, work, flip);
     return hipGetLastError();
 }
 

@@ -42,6 +42,21 @@ hipError_t launch_speed_fallback(const StepArgs& a, hipStream_t s);
 hipError_t launch_residual(const StepArgs& a, double* prev, double* partial, int loop, int check_frequency,
                            double threshold, int save, hipStream_t s);
 int residual_blocks(const StepArgs& a);
+
+// Batched launches over many instances (work lists built by the host; args
+// holds two StepArgs per instance, icur the start buffer of each instance).
+// grid: workgroups of the stencil launch (<= nwork; fewer makes the grid
+// persistent over the list).
+hipError_t launch_stencil_batch(const StepArgs* args, const int* icur, const BatchBlock* work, int nwork, int grid,
+                                int flip, bool fast_div, bool buf, double dt_explicit, int loop, hipStream_t s);
+hipError_t launch_bc_batch(const StepArgs* args, const int* icur, const BatchBlock* work, int nwork, int flip,
+                           hipStream_t s);
+hipError_t launch_residual_batch(const StepArgs* args, const int* icur, const BatchAux* aux, const BatchBlock* work,
+                                 int nwork, int flip, hipStream_t s);
+// Tiles of one instance for tile height H (strips of central columns x row chunks).
+inline int stencil_strips(int Mn) { return (Mn - 1) / kStripCols + 1; }
+inline int stencil_tiles(int L, int Mn, int H) { return stencil_strips(Mn) * ((L + H - 1) / H); }
+inline int default_tile_rows() { return 8; }
 hipError_t launch_max_speed(const StepArgs& a, unsigned long long* out, hipStream_t s);
 hipError_t launch_phys_fields(const StepArgs& a, double* out, hipStream_t s);
 hipError_t launch_integrals(const StepArgs& a, double* rowsum, double* out, hipStream_t s);

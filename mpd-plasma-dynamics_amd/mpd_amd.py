@@ -40,7 +40,9 @@ EXPORTS = ["mpd_create_from_file", "mpd_create_from_string", "mpd_destroy", "mpd
            "mpd_upload_state", "mpd_download", "mpd_advance", "mpd_compute_dt", "mpd_max_wave_speed",
            "mpd_solution_change", "mpd_run", "mpd_integrals", "mpd_outlet", "mpd_radial_symmetry", "mpd_grid",
            "mpd_config_check", "mpd_initial_state", "mpd_set_option", "mpd_time_steps", "mpd_probe",
-           "mpd_write_vtk", "mpd_write_vtk_fields"]
+           "mpd_write_vtk", "mpd_write_vtk_fields", "mpd_batch_create", "mpd_batch_destroy", "mpd_batch_size",
+           "mpd_batch_instance", "mpd_batch_set_option", "mpd_batch_autotune", "mpd_batch_tile_rows",
+           "mpd_batch_run"]
 
 _lib = None
 _DP = C.POINTER(C.c_double)
@@ -76,8 +78,17 @@ def load():
     L.mpd_initial_state.argtypes = [C.c_char_p, C.POINTER(_DP), C.POINTER(_DP)]
     L.mpd_set_option.argtypes = [vp, C.c_char_p, C.c_double]
     L.mpd_time_steps.argtypes = [vp, C.c_int, C.c_int, _DP, _DP, _DP]
+    L.mpd_batch_create.argtypes = [C.POINTER(C.c_char_p), C.c_int, C.POINTER(vp)]
+    L.mpd_batch_destroy.argtypes = [vp]
+    L.mpd_batch_destroy.restype = None
+    L.mpd_batch_size.argtypes = [vp, C.POINTER(C.c_int)]
+    L.mpd_batch_instance.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.mpd_batch_set_option.argtypes = [vp, C.c_char_p, C.c_double]
+    L.mpd_batch_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_int]
+    L.mpd_batch_tile_rows.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.mpd_batch_run.argtypes = [vp, C.c_int, C.POINTER(RunResult)]
     for name in EXPORTS:
-        if name not in ("mpd_destroy", "mpd_last_error"):
+        if name not in ("mpd_destroy", "mpd_last_error", "mpd_batch_destroy"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -118,21 +129,25 @@ def write_vtk_fields(config, phys, path, binary=True):
 class Solver:
     """Device-resident LF solver built from a config (path or YAML text)."""
 
-    def __init__(self, config=None, *, path=None):
+    def __init__(self, config=None, *, path=None, _handle=None, _owner=None):
         L = load()
         h = C.c_void_p()
-        if path is not None:
+        if _handle is not None:
+            h = _handle
+        elif path is not None:
             _check(L.mpd_create_from_file(str(path).encode(), C.byref(h)))
         else:
             _check(L.mpd_create_from_string(config.encode(), C.byref(h)))
         self._h = h
+        self._owner = _owner  # a Batch keeps its instances alive
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _check(L.mpd_dims(h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         self.L_max, self.M_max, self.rows, self.cols = a.value, b.value, c.value, d.value
 
     def close(self):
         if getattr(self, "_h", None):
-            load().mpd_destroy(self._h)
+            if getattr(self, "_owner", None) is None:
+                load().mpd_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -229,3 +244,70 @@ class Solver:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         _check(load().mpd_time_steps(self._h, int(steps), int(warmup), C.byref(a), C.byref(b), C.byref(c)))
         return dict(total_ms=a.value, stencil_ms=b.value, bc_ms=c.value)
+
+
+class Batch:
+    """Many independent solver instances (configs of any grid size) advanced
+    together by the batched kernels; `instance(i)` is a Solver view of one."""
+
+    def __init__(self, configs):
+        L = load()
+        texts = [c.encode() for c in configs]
+        arr = (C.c_char_p * len(texts))(*texts)
+        h = C.c_void_p()
+        _check(L.mpd_batch_create(arr, len(texts), C.byref(h)))
+        self._h = h
+        self.n = len(texts)
+        self._views = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for v in self._views.values():
+                v._h = None
+            self._views = {}
+            load().mpd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def instance(self, i):
+        if i not in self._views or self._views[i]._h is None:
+            h = C.c_void_p()
+            _check(load().mpd_batch_instance(self._h, int(i), C.byref(h)))
+            self._views[i] = Solver(_handle=h, _owner=self)
+        return self._views[i]
+
+    def set_option(self, key, value):
+        _check(load().mpd_batch_set_option(self._h, key.encode(), float(value)))
+
+    def autotune(self, reps=20):
+        buf = C.create_string_buffer(4096)
+        _check(load().mpd_batch_autotune(self._h, int(reps), buf, len(buf)))
+        return buf.value.decode()
+
+    def tile_rows(self, i):
+        v = C.c_int()
+        _check(load().mpd_batch_tile_rows(self._h, int(i), C.byref(v)))
+        return v.value
+
+    def warm_start(self, i, u0):
+        """Replace instance i's state by u0 (8 arrays, physical = f(u0)) and
+        take it as the convergence snapshot."""
+        s = self.instance(i)
+        s.upload(u0)
+        s.solution_change(save_prev=True)
+
+    def run(self, max_steps=0):
+        res = (RunResult * self.n)()
+        _check(load().mpd_batch_run(self._h, int(max_steps), res))
+        return [r.as_dict() for r in res]
