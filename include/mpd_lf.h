@@ -60,6 +60,13 @@ int mpd_dims(const mpd_solver* s, int* L_max, int* M_max, int* rows, int* cols);
  * (src/main.cpp:147-148).  Ghost cells are taken as zero. */
 int mpd_upload_state(mpd_solver* s, const double* const* u0, const double* const* phys);
 
+/* Reset the time-loop variables to those of a fresh run (src/main.cpp:191-209):
+ * t = 0, dt = time.dt, step count 0, no convergence check yet.  With
+ * mpd_upload_state of the IC (phys given) and mpd_solution_change(s, 1, ...)
+ * (the init snapshot, src/main.cpp:150-152) the next mpd_run is a cold start
+ * identical to a freshly created solver's. */
+int mpd_reset_loop(mpd_solver* s);
+
 /* Copy the state out: u0 (8 arrays) and/or the physical arrays (10), either
  * may be NULL.  Physical values are those Fields holds after the last step. */
 int mpd_download(mpd_solver* s, double* const* u0, double* const* phys);

@@ -1,0 +1,579 @@
+// nn_kernels.hip -- gfx950 kernels of the learned warm start: a BF16 conv
+// encoder-decoder (3x3 convolutions, FP32 accumulate on the matrix cores)
+// with its training kernels.  C ABI in include/mpd_nn.h; every pointer is a
+// device pointer, every tensor NHWC (batch, z rows, r columns, channels),
+// activations and MFMA operands bfloat16, pre-activations and gradients FP32.
+//
+// The three convolution passes are one implicit GEMM C[m][n] = sum_k A(m,k)
+// B(n,k) with mode-specific operand gathers:
+//   forward      m = pixel, n = output channel, k = (tap, input channel)
+//   data grad    m = pixel, n = input channel,  k = (tap, output channel)
+//   weight grad  m = output channel, n = (tap, input channel), k = pixel
+//                (split over pixel ranges; partial sums reduced in a fixed
+//                order, so the result is deterministic)
+// A 256-thread workgroup computes a 64 x 64 tile of C: operand tiles of
+// 64 x 32 staged through LDS, each wave a 32 x 32 sub-tile as 2 x 2
+// v_mfma_f32_16x16x16_bf16 accumulators.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+
+#include "../../include/mpd_nn.h"
+
+namespace {
+
+thread_local std::string g_nn_error;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Round to nearest even (finite inputs; NaN stays NaN).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+enum { MODE_FWD = 0, MODE_DGRAD = 1, MODE_WGRAD = 2 };
+
+struct ConvArgs {
+    const uint16_t* x;   // input activations [B][H][W][Cin]
+    const uint16_t* w;   // weights [Cout][3][3][Cin]
+    const uint16_t* dz;  // pre-activation gradient [B][H][W][Cout]
+    const float* bias;   // [Cout] (forward)
+    float* out;          // forward: z [P][Cout]; data grad: dx [P][Cin]; weight grad: partials [S][Cout][9 Cin]
+    int B, H, W, Cin, Cout;
+    int M, N, K;
+    int kchunk;  // weight grad: pixels per split
+};
+
+constexpr int BM = 64, BN = 64, BK = 32, LDP = BK + 8;  // LDS row pitch (bf16)
+
+// Input activation at pixel m shifted by tap (ky, kx) - 1, channel ci; zero
+// outside the image (same padding).
+__device__ __forceinline__ uint16_t gather_x(const ConvArgs& a, int m, int tap, int ci) {
+    const int hw = a.H * a.W;
+    const int b = m / hw, r = m - b * hw;
+    const int y = r / a.W + tap / 3 - 1, x = r % a.W + tap % 3 - 1;
+    if (y < 0 || y >= a.H || x < 0 || x >= a.W) return 0;
+    return a.x[((int64_t)(b * a.H + y) * a.W + x) * a.Cin + ci];
+}
+
+template <int MODE>
+__device__ __forceinline__ uint16_t load_a(const ConvArgs& a, int m, int k) {
+    if (m >= a.M || k >= a.K) return 0;
+    if (MODE == MODE_FWD) {
+        const int tap = k / a.Cin, ci = k - tap * a.Cin;
+        return gather_x(a, m, tap, ci);
+    } else if (MODE == MODE_DGRAD) {
+        // dx[y][x][ci] = sum dz[y + 1 - ky][x + 1 - kx][co] w[co][ky][kx][ci]
+        const int tap = k / a.Cout, co = k - tap * a.Cout;
+        const int hw = a.H * a.W;
+        const int b = m / hw, r = m - b * hw;
+        const int y = r / a.W + 1 - tap / 3, x = r % a.W + 1 - tap % 3;
+        if (y < 0 || y >= a.H || x < 0 || x >= a.W) return 0;
+        return a.dz[((int64_t)(b * a.H + y) * a.W + x) * a.Cout + co];
+    } else {
+        // A(co, pixel) = dz[pixel][co]
+        return a.dz[(int64_t)k * a.Cout + m];
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ uint16_t load_b(const ConvArgs& a, int n, int k) {
+    if (n >= a.N || k >= a.K) return 0;
+    if (MODE == MODE_FWD) {
+        return a.w[(int64_t)n * a.K + k];
+    } else if (MODE == MODE_DGRAD) {
+        const int tap = k / a.Cout, co = k - tap * a.Cout;
+        return a.w[((int64_t)co * 9 + tap) * a.Cin + n];
+    } else {
+        // B((tap, ci), pixel) = input at pixel shifted by tap
+        const int tap = n / a.Cin, ci = n - tap * a.Cin;
+        return gather_x(a, k, tap, ci);
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDP];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDP];
+    const int t = (int)threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = (int)blockIdx.x * BM, n0 = (int)blockIdx.y * BN;
+    int k_begin = 0, k_end = a.K;
+    if (MODE == MODE_WGRAD) {
+        k_begin = (int)blockIdx.z * a.kchunk;
+        k_end = min(a.K, k_begin + a.kchunk);
+    }
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int row = lane & 15, kq = (lane >> 4) * 4;
+    for (int k0 = k_begin; k0 < k_end; k0 += BK) {
+        // Stage the 64 x 32 operand tiles: element e = t + 256 j of each,
+        // row e / 32, column e % 32 (consecutive threads, consecutive k).
+        uint16_t va[8], vb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int e = t + 256 * j;
+            const int r = e >> 5, c = e & 31;
+            const int k = k0 + c;
+            va[j] = (k < k_end) ? load_a<MODE>(a, m0 + r, k) : (uint16_t)0;
+            vb[j] = (k < k_end) ? load_b<MODE>(a, n0 + r, k) : (uint16_t)0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int e = t + 256 * j;
+            const int r = e >> 5, c = e & 31;
+            As[r * LDP + c] = va[j];
+            Bs[r * LDP + c] = vb[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 16) {
+            bf16x4 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const uint16_t* pa = &As[(wm * 32 + i * 16 + row) * LDP + ks + kq];
+                const uint16_t* pb = &Bs[(wn * 32 + i * 16 + row) * LDP + ks + kq];
+                fa[i] = *reinterpret_cast<const bf16x4*>(pa);
+                fb[i] = *reinterpret_cast<const bf16x4*>(pb);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // Lane holds C[4 (lane / 16) + jj][lane % 16] of each 16 x 16 sub-tile.
+    float* out = a.out;
+    if (MODE == MODE_WGRAD) out += (int64_t)blockIdx.z * a.M * a.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int m = m0 + wm * 32 + i * 16 + kq + jj;
+                const int n = n0 + wn * 32 + j * 16 + row;
+                if (m < a.M && n < a.N) {
+                    float v = acc[i][j][jj];
+                    if (MODE == MODE_FWD && a.bias) v += a.bias[n];
+                    out[(int64_t)m * a.N + n] = v;
+                }
+            }
+}
+
+// dw[i] = sum over splits s (in order) of part[s][i].
+__global__ void reduce_splits(const float* part, float* dw, int64_t n, int S) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int z = 0; z < S; ++z) s += part[(int64_t)z * n + i];
+        dw[i] = s;
+    }
+}
+
+// Bias gradient: db[c] = sum over pixels of dz[p][c] (one block per channel,
+// fixed-order tree).
+__global__ void __launch_bounds__(256) bias_grad(const float* dz, float* db, int P, int C) {
+    const int c = (int)blockIdx.x;
+    float s = 0.f;
+    for (int p = (int)threadIdx.x; p < P; p += 256) s += dz[(int64_t)p * C + c];
+    __shared__ float sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = sh[0];
+}
+
+__device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
+
+#define GRID_STRIDE(i, n) \
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+__global__ void silu_fwd(const float* z, uint16_t* a, int64_t n) {
+    GRID_STRIDE(i, n) {
+        const float v = z[i];
+        a[i] = f2bf(v * sigmoid(v));
+    }
+}
+
+// dz = da * silu'(z); written as bf16 (matrix-core operand) and FP32 (bias).
+__global__ void silu_bwd(const float* z, const float* da, uint16_t* dzb, float* dzf, int64_t n) {
+    GRID_STRIDE(i, n) {
+        const float v = z[i];
+        const float s = sigmoid(v);
+        const float d = da[i] * (s * (1.f + v * (1.f - s)));
+        if (dzf) dzf[i] = d;
+        dzb[i] = f2bf(d);
+    }
+}
+
+__global__ void to_bf16(const float* x, uint16_t* y, int64_t n) {
+    GRID_STRIDE(i, n) y[i] = f2bf(x[i]);
+}
+
+__global__ void avgpool2_fwd(const uint16_t* x, uint16_t* y, int B, int H, int W, int C) {
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t n = (int64_t)B * Ho * Wo * C;
+    GRID_STRIDE(i, n) {
+        const int c = (int)(i % C);
+        int64_t r = i / C;
+        const int xo = (int)(r % Wo);
+        r /= Wo;
+        const int yo = (int)(r % Ho);
+        const int b = (int)(r / Ho);
+        const int64_t base = ((int64_t)(b * H + 2 * yo) * W + 2 * xo) * C + c;
+        const float s = bf2f(x[base]) + bf2f(x[base + C]) + bf2f(x[base + (int64_t)W * C]) +
+                        bf2f(x[base + (int64_t)W * C + C]);
+        y[i] = f2bf(0.25f * s);
+    }
+}
+
+__global__ void avgpool2_bwd(const float* dy, float* dx, int B, int H, int W, int C) {
+    const int64_t n = (int64_t)B * H * W * C;
+    const int Ho = H / 2, Wo = W / 2;
+    GRID_STRIDE(i, n) {
+        const int c = (int)(i % C);
+        int64_t r = i / C;
+        const int x = (int)(r % W);
+        r /= W;
+        const int y = (int)(r % H);
+        const int b = (int)(r / H);
+        dx[i] = 0.25f * dy[((int64_t)(b * Ho + y / 2) * Wo + x / 2) * C + c];
+    }
+}
+
+// Nearest-neighbour x2: y[b][Y][X][c] = x[b][Y/2][X/2][c] for Y < 2H, X < 2W.
+__global__ void upsample2_fwd(const uint16_t* x, uint16_t* y, int B, int H, int W, int C) {
+    const int Ho = 2 * H, Wo = 2 * W;
+    const int64_t n = (int64_t)B * Ho * Wo * C;
+    GRID_STRIDE(i, n) {
+        const int c = (int)(i % C);
+        int64_t r = i / C;
+        const int X = (int)(r % Wo);
+        r /= Wo;
+        const int Y = (int)(r % Ho);
+        const int b = (int)(r / Ho);
+        y[i] = x[((int64_t)(b * H + Y / 2) * W + X / 2) * C + c];
+    }
+}
+
+__global__ void upsample2_bwd(const float* dy, float* dx, int B, int H, int W, int C) {
+    const int Wo = 2 * W;
+    const int64_t n = (int64_t)B * H * W * C;
+    GRID_STRIDE(i, n) {
+        const int c = (int)(i % C);
+        int64_t r = i / C;
+        const int x = (int)(r % W);
+        r /= W;
+        const int y = (int)(r % H);
+        const int b = (int)(r / H);
+        const int64_t base = ((int64_t)(b * 2 * H + 2 * y) * Wo + 2 * x) * C + c;
+        dx[i] = dy[base] + dy[base + C] + dy[base + (int64_t)Wo * C] + dy[base + (int64_t)Wo * C + C];
+    }
+}
+
+// y[p] = [a[p] (Ca channels), b[p] (Cb channels)].
+__global__ void concat_fwd(const uint16_t* a, int Ca, const uint16_t* b, int Cb, uint16_t* y, int64_t P) {
+    const int C = Ca + Cb;
+    const int64_t n = P * C;
+    GRID_STRIDE(i, n) {
+        const int64_t p = i / C;
+        const int c = (int)(i - p * C);
+        y[i] = (c < Ca) ? a[p * Ca + c] : b[p * Cb + (c - Ca)];
+    }
+}
+
+// da (+)= dy[:, :Ca], db (+)= dy[:, Ca:]; accumulate when acc_a / acc_b.
+__global__ void concat_bwd(const float* dy, float* da, int Ca, int acc_a, float* db, int Cb, int acc_b, int64_t P) {
+    const int C = Ca + Cb;
+    const int64_t n = P * C;
+    GRID_STRIDE(i, n) {
+        const int64_t p = i / C;
+        const int c = (int)(i - p * C);
+        const float v = dy[i];
+        if (c < Ca) {
+            float* d = &da[p * Ca + c];
+            *d = acc_a ? *d + v : v;
+        } else {
+            float* d = &db[p * Cb + (c - Ca)];
+            *d = acc_b ? *d + v : v;
+        }
+    }
+}
+
+__global__ void add_inplace(float* y, const float* x, int64_t n) {
+    GRID_STRIDE(i, n) y[i] += x[i];
+}
+
+// Mean squared error over n values: dy = 2 (y - t) / n; per-block partial
+// sums of (y - t)^2 into part[blockIdx.x] (the host sums them).
+__global__ void __launch_bounds__(256) mse(const float* y, const float* t, float* dy, float* part, int64_t n) {
+    float s = 0.f;
+    const float sc = 2.f / (float)n;
+    GRID_STRIDE(i, n) {
+        const float d = y[i] - t[i];
+        s += d * d;
+        if (dy) dy[i] = sc * d;
+    }
+    __shared__ float sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; writes the bf16 copy.
+__global__ void adam(float* w, uint16_t* wb, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                     float b2, float eps, float c1, float c2) {
+    GRID_STRIDE(i, n) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (wb) wb[i] = f2bf(wi);
+    }
+}
+
+// dst[j] = src[idx[j]] (rows of row_len floats).
+__global__ void gather_rows(const float* src, const int* idx, float* dst, int nrows, int64_t row_len) {
+    const int64_t n = (int64_t)nrows * row_len;
+    GRID_STRIDE(i, n) {
+        const int64_t j = i / row_len;
+        dst[i] = src[(int64_t)idx[j] * row_len + (i - j * row_len)];
+    }
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+
+int finish(hipError_t e) {
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_nn_error = hipGetErrorString(e);
+        return MPD_NN_E_DEVICE;
+    }
+    return MPD_NN_OK;
+}
+
+int bad(const char* msg) {
+    g_nn_error = msg;
+    return MPD_NN_E_ARG;
+}
+
+bool conv_dims_ok(int B, int H, int W, int Cin, int Cout) {
+    return B > 0 && H > 0 && W > 0 && Cin > 0 && Cout > 0 && (int64_t)B * H * W < (1LL << 30) &&
+           (int64_t)9 * Cin < (1 << 20) && (int64_t)9 * Cout < (1 << 20);
+}
+
+int wgrad_splits(int P) { return std::max(1, std::min(64, P / 2048)); }
+int wgrad_chunk(int P, int S) { return ((P + S - 1) / S + BK - 1) / BK * BK; }
+
+}  // namespace
+
+extern "C" {
+
+const char* mpd_nn_last_error(void) { return g_nn_error.c_str(); }
+
+int mpd_nn_malloc(void** p, long long bytes) {
+    if (!p || bytes < 0) return bad("malloc: bad argument");
+    *p = nullptr;
+    return finish(hipMalloc(p, (size_t)std::max(bytes, 1LL)));
+}
+
+int mpd_nn_free(void* p) { return finish(p ? hipFree(p) : hipSuccess); }
+
+int mpd_nn_memcpy_h2d(void* dst, const void* src, long long bytes) {
+    if (!dst || !src || bytes < 0) return bad("memcpy_h2d: bad argument");
+    return finish(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+}
+
+int mpd_nn_memcpy_d2h(void* dst, const void* src, long long bytes) {
+    if (!dst || !src || bytes < 0) return bad("memcpy_d2h: bad argument");
+    return finish(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+}
+
+int mpd_nn_memset(void* p, int value, long long bytes) {
+    if (!p || bytes < 0) return bad("memset: bad argument");
+    return finish(hipMemset(p, value, (size_t)bytes));
+}
+
+int mpd_nn_synchronize(void) { return finish(hipDeviceSynchronize()); }
+
+int mpd_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, float* z, int B, int H, int W,
+                       int Cin, int Cout, void* stream) {
+    if (!x || !w || !z || !conv_dims_ok(B, H, W, Cin, Cout)) return bad("conv3x3_fwd: bad argument");
+    ConvArgs a{};
+    a.x = x;
+    a.w = w;
+    a.bias = bias;
+    a.out = z;
+    a.B = B, a.H = H, a.W = W, a.Cin = Cin, a.Cout = Cout;
+    a.M = B * H * W;
+    a.N = Cout;
+    a.K = 9 * Cin;
+    const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
+    hipLaunchKernelGGL(conv_gemm<MODE_FWD>, g, dim3(256), 0, (hipStream_t)stream, a);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, int B, int H, int W, int Cin, int Cout,
+                            void* stream) {
+    if (!dz || !w || !dx || !conv_dims_ok(B, H, W, Cin, Cout)) return bad("conv3x3_bwd_data: bad argument");
+    ConvArgs a{};
+    a.dz = dz;
+    a.w = w;
+    a.out = dx;
+    a.B = B, a.H = H, a.W = W, a.Cin = Cin, a.Cout = Cout;
+    a.M = B * H * W;
+    a.N = Cin;
+    a.K = 9 * Cout;
+    const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
+    hipLaunchKernelGGL(conv_gemm<MODE_DGRAD>, g, dim3(256), 0, (hipStream_t)stream, a);
+    return finish(hipSuccess);
+}
+
+long long mpd_nn_wgrad_work_floats(int B, int H, int W, int Cin, int Cout) {
+    if (!conv_dims_ok(B, H, W, Cin, Cout)) return -1;
+    const int P = B * H * W;
+    return (long long)wgrad_splits(P) * Cout * 9 * Cin;
+}
+
+int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, float* work, int B, int H, int W,
+                              int Cin, int Cout, void* stream) {
+    if (!x || !dz || !dw || !work || !conv_dims_ok(B, H, W, Cin, Cout)) return bad("conv3x3_bwd_weight: bad argument");
+    ConvArgs a{};
+    a.x = x;
+    a.dz = dz;
+    a.out = work;
+    a.B = B, a.H = H, a.W = W, a.Cin = Cin, a.Cout = Cout;
+    a.M = Cout;
+    a.N = 9 * Cin;
+    a.K = B * H * W;
+    const int S = wgrad_splits(a.K);
+    a.kchunk = wgrad_chunk(a.K, S);
+    const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN), (unsigned)S);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(conv_gemm<MODE_WGRAD>, g, dim3(256), 0, s, a);
+    const int64_t n = (int64_t)a.M * a.N;
+    hipLaunchKernelGGL(reduce_splits, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)work, dw, n, S);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_bias_grad(const float* dz, float* db, int P, int C, void* stream) {
+    if (!dz || !db || P <= 0 || C <= 0) return bad("bias_grad: bad argument");
+    hipLaunchKernelGGL(bias_grad, dim3((unsigned)C), dim3(256), 0, (hipStream_t)stream, dz, db, P, C);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_silu_fwd(const float* z, uint16_t* a, long long n, void* stream) {
+    if (!z || !a || n < 0) return bad("silu_fwd: bad argument");
+    hipLaunchKernelGGL(silu_fwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, z, a, (int64_t)n);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_silu_bwd(const float* z, const float* da, uint16_t* dz_bf16, float* dz_f32, long long n, void* stream) {
+    if (!z || !da || !dz_bf16 || n < 0) return bad("silu_bwd: bad argument");
+    hipLaunchKernelGGL(silu_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, z, da, dz_bf16, dz_f32,
+                       (int64_t)n);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_to_bf16(const float* x, uint16_t* y, long long n, void* stream) {
+    if (!x || !y || n < 0) return bad("to_bf16: bad argument");
+    hipLaunchKernelGGL(to_bf16, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, x, y, (int64_t)n);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_avgpool2_fwd(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, void* stream) {
+    if (!x || !y || B <= 0 || H < 2 || W < 2 || C <= 0 || (H & 1) || (W & 1)) return bad("avgpool2_fwd: bad argument");
+    const int64_t n = (int64_t)B * (H / 2) * (W / 2) * C;
+    hipLaunchKernelGGL(avgpool2_fwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, x, y, B, H, W, C);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_avgpool2_bwd(const float* dy, float* dx, int B, int H, int W, int C, void* stream) {
+    if (!dy || !dx || B <= 0 || H < 2 || W < 2 || C <= 0 || (H & 1) || (W & 1)) return bad("avgpool2_bwd: bad argument");
+    const int64_t n = (int64_t)B * H * W * C;
+    hipLaunchKernelGGL(avgpool2_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, dy, dx, B, H, W, C);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_upsample2_fwd(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, void* stream) {
+    if (!x || !y || B <= 0 || H <= 0 || W <= 0 || C <= 0) return bad("upsample2_fwd: bad argument");
+    const int64_t n = (int64_t)B * 4 * H * W * C;
+    hipLaunchKernelGGL(upsample2_fwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, x, y, B, H, W, C);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_upsample2_bwd(const float* dy, float* dx, int B, int H, int W, int C, void* stream) {
+    if (!dy || !dx || B <= 0 || H <= 0 || W <= 0 || C <= 0) return bad("upsample2_bwd: bad argument");
+    const int64_t n = (int64_t)B * H * W * C;
+    hipLaunchKernelGGL(upsample2_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, dy, dx, B, H, W, C);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_concat_fwd(const uint16_t* a, int Ca, const uint16_t* b, int Cb, uint16_t* y, long long P, void* stream) {
+    if (!a || !b || !y || Ca <= 0 || Cb <= 0 || P < 0) return bad("concat_fwd: bad argument");
+    hipLaunchKernelGGL(concat_fwd, dim3(blocks_for(P * (Ca + Cb))), dim3(256), 0, (hipStream_t)stream, a, Ca, b, Cb,
+                       y, (int64_t)P);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_concat_bwd(const float* dy, float* da, int Ca, int acc_a, float* db, int Cb, int acc_b, long long P,
+                      void* stream) {
+    if (!dy || !da || !db || Ca <= 0 || Cb <= 0 || P < 0) return bad("concat_bwd: bad argument");
+    hipLaunchKernelGGL(concat_bwd, dim3(blocks_for(P * (Ca + Cb))), dim3(256), 0, (hipStream_t)stream, dy, da, Ca,
+                       acc_a, db, Cb, acc_b, (int64_t)P);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_add(float* y, const float* x, long long n, void* stream) {
+    if (!y || !x || n < 0) return bad("add: bad argument");
+    hipLaunchKernelGGL(add_inplace, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, y, x, (int64_t)n);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_mse_blocks(long long n) { return (int)std::min<long long>(1024, std::max<long long>(1, (n + 255) / 256)); }
+
+int mpd_nn_mse(const float* y, const float* t, float* dy, float* part, long long n, void* stream) {
+    if (!y || !t || !part || n <= 0) return bad("mse: bad argument");
+    hipLaunchKernelGGL(mse, dim3((unsigned)mpd_nn_mse_blocks(n)), dim3(256), 0, (hipStream_t)stream, y, t, dy, part,
+                       (int64_t)n);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_adam(float* w, uint16_t* w_bf16, const float* g, float* m, float* v, long long n, float lr, float b1,
+                float b2, float eps, int step, void* stream) {
+    if (!w || !g || !m || !v || n < 0 || step < 1) return bad("adam: bad argument");
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, w, w_bf16, g, m, v, (int64_t)n,
+                       lr, b1, b2, eps, c1, c2);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_gather_rows(const float* src, const int* idx, float* dst, int nrows, long long row_len, void* stream) {
+    if (!src || !idx || !dst || nrows < 0 || row_len <= 0) return bad("gather_rows: bad argument");
+    hipLaunchKernelGGL(gather_rows, dim3(blocks_for((int64_t)nrows * row_len)), dim3(256), 0, (hipStream_t)stream, src,
+                       idx, dst, nrows, (int64_t)row_len);
+    return finish(hipSuccess);
+}
+
+}  // extern "C"

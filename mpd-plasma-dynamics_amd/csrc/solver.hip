@@ -46,6 +46,13 @@ int mpd_dims(const mpd_solver* s, int* L_max, int* M_max, int* rows, int* cols) 
     });
 }
 
+int mpd_reset_loop(mpd_solver* s) {
+    return Guard([&] {
+        Need(s);
+        s->ResetLoop();
+    });
+}
+
 int mpd_upload_state(mpd_solver* s, const double* const* u0, const double* const* phys) {
     return Guard([&] {
         Need(s);

@@ -422,6 +422,18 @@ struct mpd_solver {
         UploadFields(prev, p7, 7);
     }
 
+    // The loop variables of a fresh run (src/main.cpp:191-209): t = 0,
+    // dt = time.dt, no steps taken, no convergence check yet.
+    void ResetLoop() {
+        t = 0.0;
+        dt = cfg.dt;
+        step = 0;
+        checked_step = 0;
+        converged = false;
+        change = -1.0;
+        speed_src = 0;
+    }
+
     void UploadState(const double* const* u0, const double* const* phys) {
         HIPCHK(hipStreamSynchronize(stream));
         UploadFields(state[cur], u0, 8);

@@ -37,7 +37,7 @@ class RunResult(C.Structure):
 
 # Every symbol include/mpd_lf.h declares.
 EXPORTS = ["mpd_create_from_file", "mpd_create_from_string", "mpd_destroy", "mpd_last_error", "mpd_dims",
-           "mpd_upload_state", "mpd_download", "mpd_advance", "mpd_compute_dt", "mpd_max_wave_speed",
+           "mpd_upload_state", "mpd_reset_loop", "mpd_download", "mpd_advance", "mpd_compute_dt", "mpd_max_wave_speed",
            "mpd_solution_change", "mpd_run", "mpd_integrals", "mpd_outlet", "mpd_radial_symmetry", "mpd_grid",
            "mpd_config_check", "mpd_initial_state", "mpd_set_option", "mpd_time_steps", "mpd_probe",
            "mpd_write_vtk", "mpd_write_vtk_fields", "mpd_batch_create", "mpd_batch_destroy", "mpd_batch_size",
@@ -65,6 +65,7 @@ def load():
     L.mpd_dims.argtypes = [vp] + [C.POINTER(C.c_int)] * 4
     L.mpd_upload_state.argtypes = [vp, C.POINTER(_DP), C.POINTER(_DP)]
     L.mpd_download.argtypes = [vp, C.POINTER(_DP), C.POINTER(_DP)]
+    L.mpd_reset_loop.argtypes = [vp]
     L.mpd_advance.argtypes = [vp, C.c_double]
     L.mpd_compute_dt.argtypes = [vp, C.c_double, _DP]
     L.mpd_max_wave_speed.argtypes = [vp, _DP]
@@ -179,6 +180,18 @@ class Solver:
             if a.shape != (self.rows, self.cols):
                 raise ValueError(f"array shape {a.shape} != {(self.rows, self.cols)}")
         _check(load().mpd_upload_state(self._h, _ptr_array(ua), _ptr_array(pa) if pa else None))
+
+    def reset_loop(self):
+        """Loop variables of a fresh run: t = 0, dt = time.dt, no steps."""
+        _check(load().mpd_reset_loop(self._h))
+
+    def cold_restart(self, config):
+        """Back to the config's initial condition and a fresh time loop (the
+        same run as a newly created solver's)."""
+        u0, ph = initial_state(config)
+        self.upload(u0, ph)
+        self.reset_loop()
+        self.solution_change(save_prev=True)
 
     def advance(self, dt):
         _check(load().mpd_advance(self._h, float(dt)))

@@ -1,0 +1,471 @@
+"""Learned warm start for steady-state sweeps of the LF solver.
+
+A BF16 convolutional encoder-decoder (U-Net: `levels` resolution levels,
+`base` channels at the finest, doubling per level, 3x3 convolutions, SiLU,
+2x2 average pooling, nearest x2 upsampling with skip concatenation, FP32
+accumulation) maps the sweep parameters of a case -- inflow density,
+inflow velocity, B_phi amplitude and the two wall-shape amplitudes of the
+converging-diverging channel -- to the 8 steady-state physical fields on the
+solver's grid.  Every forward and backward operation runs on the hand-written
+gfx950 kernels of libmpd_nn.so (include/mpd_nn.h), on device memory that
+library allocates; the host side is numpy.  The prediction becomes the initial state of the
+plain FP64 LF loop, which relaxes it to the convergence threshold: the solver
+is the judge, and a case whose residual stalls falls back to a cold start.
+
+No counterpart in the reference: its runs start from the expression or VTK
+initial condition (src/main.cpp:117-148).
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NN_LIB_PATH = os.path.join(HERE, "libmpd_nn.so")
+
+# Input features per pixel: the 5 parameters (scaled to [-1, 1]), z, the
+# radial grid fraction and the radius.
+N_FEATURES = 8
+# Output channels: the physical fields the warm start sets (Fields member
+# names), in this order.
+OUT_FIELDS = ["rho", "v_z", "v_r", "v_phi", "e", "H_z", "H_r", "H_phi"]
+PHYS_INDEX = {"rho": 0, "v_z": 1, "v_r": 2, "v_phi": 3, "e": 4, "p": 5, "P": 6, "H_z": 7, "H_r": 8, "H_phi": 9}
+# Parameter ranges of the sweeps: rho_in, u_z_in, B_phi amplitude, inner and
+# outer wall amplitudes.
+PARAM_LO = np.array([0.5, 0.05, 0.05, 0.0, 0.0])
+PARAM_HI = np.array([2.0, 0.3, 0.5, 0.1, 0.1])
+
+_nn = None
+_vp = C.c_void_p
+
+
+def nn_lib():
+    """Load libmpd_nn.so (raises if it was not built: no CPU fallback)."""
+    global _nn
+    if _nn is not None:
+        return _nn
+    if not os.path.exists(NN_LIB_PATH):
+        raise RuntimeError(f"{NN_LIB_PATH} is missing: build it first")
+    L = C.CDLL(NN_LIB_PATH)
+    i, ll, f = C.c_int, C.c_longlong, C.c_float
+    sig = {
+        "mpd_nn_conv3x3_fwd": [_vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
+        "mpd_nn_conv3x3_bwd_data": [_vp, _vp, _vp, i, i, i, i, i, _vp],
+        "mpd_nn_conv3x3_bwd_weight": [_vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
+        "mpd_nn_wgrad_work_floats": [i, i, i, i, i],
+        "mpd_nn_bias_grad": [_vp, _vp, i, i, _vp],
+        "mpd_nn_silu_fwd": [_vp, _vp, ll, _vp],
+        "mpd_nn_silu_bwd": [_vp, _vp, _vp, _vp, ll, _vp],
+        "mpd_nn_to_bf16": [_vp, _vp, ll, _vp],
+        "mpd_nn_avgpool2_fwd": [_vp, _vp, i, i, i, i, _vp],
+        "mpd_nn_avgpool2_bwd": [_vp, _vp, i, i, i, i, _vp],
+        "mpd_nn_upsample2_fwd": [_vp, _vp, i, i, i, i, _vp],
+        "mpd_nn_upsample2_bwd": [_vp, _vp, i, i, i, i, _vp],
+        "mpd_nn_concat_fwd": [_vp, i, _vp, i, _vp, ll, _vp],
+        "mpd_nn_concat_bwd": [_vp, _vp, i, i, _vp, i, i, ll, _vp],
+        "mpd_nn_add": [_vp, _vp, ll, _vp],
+        "mpd_nn_mse_blocks": [ll],
+        "mpd_nn_mse": [_vp, _vp, _vp, _vp, ll, _vp],
+        "mpd_nn_adam": [_vp, _vp, _vp, _vp, _vp, ll, f, f, f, f, i, _vp],
+        "mpd_nn_gather_rows": [_vp, _vp, _vp, i, ll, _vp],
+        "mpd_nn_malloc": [C.POINTER(_vp), ll],
+        "mpd_nn_free": [_vp],
+        "mpd_nn_memcpy_h2d": [_vp, _vp, ll],
+        "mpd_nn_memcpy_d2h": [_vp, _vp, ll],
+        "mpd_nn_memset": [_vp, i, ll],
+        "mpd_nn_synchronize": [],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_longlong if name == "mpd_nn_wgrad_work_floats" else C.c_int
+    L.mpd_nn_last_error.restype = C.c_char_p
+    _nn = L
+    return L
+
+
+# Every symbol include/mpd_nn.h declares.
+NN_EXPORTS = ["mpd_nn_last_error", "mpd_nn_conv3x3_fwd", "mpd_nn_conv3x3_bwd_data", "mpd_nn_wgrad_work_floats",
+              "mpd_nn_conv3x3_bwd_weight", "mpd_nn_bias_grad", "mpd_nn_silu_fwd", "mpd_nn_silu_bwd",
+              "mpd_nn_to_bf16", "mpd_nn_avgpool2_fwd", "mpd_nn_avgpool2_bwd", "mpd_nn_upsample2_fwd",
+              "mpd_nn_upsample2_bwd", "mpd_nn_concat_fwd", "mpd_nn_concat_bwd", "mpd_nn_add", "mpd_nn_mse_blocks",
+              "mpd_nn_mse", "mpd_nn_adam", "mpd_nn_gather_rows", "mpd_nn_malloc", "mpd_nn_free",
+              "mpd_nn_memcpy_h2d", "mpd_nn_memcpy_d2h", "mpd_nn_memset", "mpd_nn_synchronize"]
+
+
+def _ck(rc):
+    if rc != 0:
+        raise RuntimeError(f"libmpd_nn: [{rc}] {nn_lib().mpd_nn_last_error().decode(errors='replace')}")
+
+
+def _p(t):
+    return C.c_void_p(t.ptr)
+
+
+def _stream():
+    # Every kernel of this module runs on the null stream, in call order.
+    return None
+
+
+def bf16_bits(x):
+    """bfloat16 bit patterns (uint16) of float values, rounded to nearest
+    even (the rounding of the kernels' FP32 -> bf16 conversion)."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+
+
+def bf16_to_float(h):
+    return (np.asarray(h, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+class DeviceArray:
+    """A device buffer allocated by libmpd_nn.so: 'f32' (float32), 'bf16'
+    (bfloat16 bit patterns) or 'i32' (int32), C-contiguous."""
+
+    _ITEM = {"f32": 4, "bf16": 2, "i32": 4}
+
+    def __init__(self, shape, kind="f32"):
+        if isinstance(shape, (int, np.integer)):
+            shape = (shape,)
+        self.shape = tuple(int(v) for v in shape)
+        self.kind = kind
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * self._ITEM[kind]
+        p = C.c_void_p()
+        _ck(nn_lib().mpd_nn_malloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            try:
+                nn_lib().mpd_nn_free(C.c_void_p(self.ptr))
+            except Exception:
+                pass
+            self.ptr = None
+
+    def numel(self):
+        return self.size
+
+    @classmethod
+    def from_numpy(cls, a, kind="f32"):
+        d = cls(np.shape(a), kind)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        if self.kind == "f32":
+            h = np.ascontiguousarray(a, dtype=np.float32)
+        elif self.kind == "bf16":
+            h = bf16_bits(a)
+        else:
+            h = np.ascontiguousarray(a, dtype=np.int32)
+        if h.size != self.size:
+            raise ValueError(f"upload of {h.size} values into a buffer of {self.size}")
+        _ck(nn_lib().mpd_nn_memcpy_h2d(C.c_void_p(self.ptr), h.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def download(self):
+        """Host copy (float32 values for 'f32' and 'bf16', int32 for 'i32')."""
+        dt = {"f32": np.float32, "bf16": np.uint16, "i32": np.int32}[self.kind]
+        h = np.empty(self.shape, dtype=dt)
+        _ck(nn_lib().mpd_nn_memcpy_d2h(h.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes))
+        return bf16_to_float(h) if self.kind == "bf16" else h
+
+    def zero(self):
+        _ck(nn_lib().mpd_nn_memset(C.c_void_p(self.ptr), 0, self.nbytes))
+
+
+def unet_layers(cin, cout, base=32, levels=4):
+    """(Cin, Cout) of every conv in execution order (encoder levels: two
+    convs each; decoder levels: two each, the first after the skip
+    concatenation; then the output head)."""
+    ch = [base * 2 ** i for i in range(levels)]
+    layers, c = [], cin
+    for i in range(levels):
+        layers += [(c, ch[i]), (ch[i], ch[i])]
+        c = ch[i]
+    for i in range(levels - 2, -1, -1):
+        layers += [(c + ch[i], ch[i]), (ch[i], ch[i])]
+        c = ch[i]
+    layers.append((c, cout))
+    return layers
+
+
+class UNet:
+    """The encoder-decoder on the GPU: FP32 master weights with bf16 copies,
+    Adam moments, and the activations of the last forward pass (kept for the
+    backward pass)."""
+
+    def __init__(self, H, W, cin=N_FEATURES, cout=len(OUT_FIELDS), base=32, levels=4, seed=0):
+        if H % 2 ** (levels - 1) or W % 2 ** (levels - 1):
+            raise ValueError(f"grid {H}x{W} must be divisible by {2 ** (levels - 1)}")
+        self.H, self.W, self.cin, self.cout, self.base, self.levels = H, W, cin, cout, base, levels
+        self.layers = unet_layers(cin, cout, base, levels)
+        rng = np.random.default_rng(seed)
+        self.w, self.b, self.wb, self.gw, self.gb = [], [], [], [], []
+        self.mw, self.vw, self.mb, self.vb = [], [], [], []
+        for k, (ci, co) in enumerate(self.layers):
+            std = math.sqrt(2.0 / (9 * ci)) * (0.1 if k == len(self.layers) - 1 else 1.0)
+            self.w.append(DeviceArray.from_numpy(rng.normal(0.0, std, size=(co, 3, 3, ci))))
+            self.b.append(DeviceArray.from_numpy(np.zeros(co)))
+            self.wb.append(DeviceArray((co, 3, 3, ci), "bf16"))
+            for lst, shape in ((self.gw, (co, 3, 3, ci)), (self.gb, (co,)), (self.mw, (co, 3, 3, ci)),
+                               (self.vw, (co, 3, 3, ci)), (self.mb, (co,)), (self.vb, (co,))):
+                d = DeviceArray(shape)
+                d.zero()
+                lst.append(d)
+        self.step = 0
+        self._sync_bf16()
+        self._B = None
+
+    # -- parameters ------------------------------------------------------
+    def _sync_bf16(self):
+        L = nn_lib()
+        for w, wb in zip(self.w, self.wb):
+            _ck(L.mpd_nn_to_bf16(_p(w), _p(wb), w.numel(), _stream()))
+
+    def state_dict(self):
+        return {"layers": self.layers, "w": [w.download() for w in self.w], "b": [b.download() for b in self.b]}
+
+    def load_params(self, ws, bs):
+        for k in range(len(self.layers)):
+            self.w[k].upload(ws[k])
+            self.b[k].upload(bs[k])
+        self._sync_bf16()
+
+    def n_params(self):
+        return sum(w.numel() + b.numel() for w, b in zip(self.w, self.b))
+
+    # -- buffers -----------------------------------------------------------
+    def _alloc(self, B):
+        if self._B == B:
+            return
+        H, W = self.H, self.W
+        self.lay = []  # per conv: dict(h, w, ci, co, a_in, z, a_out, act)
+        h, w = H, W
+        ch = [self.base * 2 ** i for i in range(self.levels)]
+        k = 0
+        self.x_in = DeviceArray((B, H, W, self.cin), "bf16")
+        self.pools, self.ups, self.cats = [], [], []
+        for i in range(self.levels):
+            for j in range(2):
+                ci, co = self.layers[k]
+                self.lay.append(dict(h=h, w=w, ci=ci, co=co, act=True,
+                                     z=DeviceArray((B, h, w, co)),
+                                     a_out=DeviceArray((B, h, w, co), "bf16")))
+                k += 1
+            if i < self.levels - 1:
+                self.pools.append(DeviceArray((B, h // 2, w // 2, ch[i]), "bf16"))
+                h, w = h // 2, w // 2
+        for i in range(self.levels - 2, -1, -1):
+            h, w = h * 2, w * 2
+            cu = self.layers[k][0] - ch[i]
+            self.ups.append(DeviceArray((B, h, w, cu), "bf16"))
+            self.cats.append(DeviceArray((B, h, w, cu + ch[i]), "bf16"))
+            for j in range(2):
+                ci, co = self.layers[k]
+                self.lay.append(dict(h=h, w=w, ci=ci, co=co, act=True,
+                                     z=DeviceArray((B, h, w, co)),
+                                     a_out=DeviceArray((B, h, w, co), "bf16")))
+                k += 1
+        ci, co = self.layers[k]
+        self.lay.append(dict(h=h, w=w, ci=ci, co=co, act=False,
+                             z=DeviceArray((B, h, w, co)), a_out=None))
+        L = nn_lib()
+        work = max(L.mpd_nn_wgrad_work_floats(B, d["h"], d["w"], d["ci"], d["co"]) for d in self.lay)
+        self.work = DeviceArray(int(work))
+        big = max(B * d["h"] * d["w"] * max(d["ci"], d["co"]) for d in self.lay)
+        # Gradient scratch: two ping-pong FP32 buffers, bf16 dz, FP32 dz, and
+        # per-skip gradient buffers.
+        self.g0 = DeviceArray(big)
+        self.g1 = DeviceArray(big)
+        self.g2 = DeviceArray(big)
+        self.dzb = DeviceArray(big, "bf16")
+        self.dzf = DeviceArray(big)
+        self.dskip = [DeviceArray(B * H // 2 ** i * W // 2 ** i * ch[i])
+                      for i in range(self.levels - 1)]
+        self.loss_part = DeviceArray(1024)
+        self._B = B
+
+    # -- forward -------------------------------------------------------------
+    def forward(self, x_bf16, B):
+        """x_bf16: (B, H, W, cin) bf16 tensor.  Returns the FP32 output
+        (B, H, W, cout) (a view of an internal buffer)."""
+        self._alloc(B)
+        L, s = nn_lib(), _stream()
+        a = x_bf16
+        k = 0
+        skips = []
+        for i in range(self.levels):
+            for j in range(2):
+                d = self.lay[k]
+                d["a_in"] = a
+                _ck(L.mpd_nn_conv3x3_fwd(_p(a), _p(self.wb[k]), _p(self.b[k]), _p(d["z"]), B, d["h"], d["w"],
+                                         d["ci"], d["co"], s))
+                _ck(L.mpd_nn_silu_fwd(_p(d["z"]), _p(d["a_out"]), d["z"].numel(), s))
+                a = d["a_out"]
+                k += 1
+            if i < self.levels - 1:
+                skips.append(a)
+                d = self.lay[k - 1]
+                _ck(L.mpd_nn_avgpool2_fwd(_p(a), _p(self.pools[i]), B, d["h"], d["w"], d["co"], s))
+                a = self.pools[i]
+        for n, i in enumerate(range(self.levels - 2, -1, -1)):
+            d = self.lay[k]
+            hh, ww = d["h"] // 2, d["w"] // 2
+            cu = d["ci"] - skips[i].shape[-1]
+            _ck(L.mpd_nn_upsample2_fwd(_p(a), _p(self.ups[n]), B, hh, ww, cu, s))
+            _ck(L.mpd_nn_concat_fwd(_p(self.ups[n]), cu, _p(skips[i]), skips[i].shape[-1], _p(self.cats[n]),
+                                    B * d["h"] * d["w"], s))
+            a = self.cats[n]
+            for j in range(2):
+                d = self.lay[k]
+                d["a_in"] = a
+                _ck(L.mpd_nn_conv3x3_fwd(_p(a), _p(self.wb[k]), _p(self.b[k]), _p(d["z"]), B, d["h"], d["w"],
+                                         d["ci"], d["co"], s))
+                _ck(L.mpd_nn_silu_fwd(_p(d["z"]), _p(d["a_out"]), d["z"].numel(), s))
+                a = d["a_out"]
+                k += 1
+        d = self.lay[k]
+        d["a_in"] = a
+        _ck(L.mpd_nn_conv3x3_fwd(_p(a), _p(self.wb[k]), _p(self.b[k]), _p(d["z"]), B, d["h"], d["w"], d["ci"],
+                                 d["co"], s))
+        return d["z"]
+
+    # -- backward ------------------------------------------------------------
+    def _conv_back(self, k, da, dx_out, B):
+        """Gradients of conv k from da = dL/d(output) (FP32 tensor); writes
+        dL/d(input) into dx_out (FP32, or None for the first conv)."""
+        L, s = nn_lib(), _stream()
+        d = self.lay[k]
+        n = B * d["h"] * d["w"] * d["co"]
+        if d["act"]:
+            _ck(L.mpd_nn_silu_bwd(_p(d["z"]), _p(da), _p(self.dzb), _p(self.dzf), n, s))
+            dzf = self.dzf
+        else:
+            _ck(L.mpd_nn_to_bf16(_p(da), _p(self.dzb), n, s))
+            dzf = da
+        _ck(L.mpd_nn_bias_grad(_p(dzf), _p(self.gb[k]), B * d["h"] * d["w"], d["co"], s))
+        _ck(L.mpd_nn_conv3x3_bwd_weight(_p(d["a_in"]), _p(self.dzb), _p(self.gw[k]), _p(self.work), B, d["h"],
+                                        d["w"], d["ci"], d["co"], s))
+        if dx_out is not None:
+            _ck(L.mpd_nn_conv3x3_bwd_data(_p(self.dzb), _p(self.wb[k]), _p(dx_out), B, d["h"], d["w"], d["ci"],
+                                          d["co"], s))
+
+    def backward(self, dy, B):
+        """dy: dL/d(output) FP32 (B, H, W, cout).  Fills the gradients."""
+        L, s = nn_lib(), _stream()
+        k = len(self.lay) - 1
+        cur, nxt, tmp = self.g0, self.g1, self.g2
+        self._conv_back(k, dy, cur, B)
+        k -= 1
+        ch = [self.base * 2 ** i for i in range(self.levels)]
+        for i in range(self.levels - 1):  # decoder levels, finest first
+            self._conv_back(k, cur, nxt, B)
+            cur, nxt = nxt, cur
+            k -= 1
+            d = self.lay[k]
+            self._conv_back(k, cur, nxt, B)
+            cur, nxt = nxt, cur
+            k -= 1
+            cu = d["ci"] - ch[i]
+            P = B * d["h"] * d["w"]
+            _ck(L.mpd_nn_concat_bwd(_p(cur), _p(tmp), cu, 0, _p(self.dskip[i]), ch[i], 0, P, s))
+            _ck(L.mpd_nn_upsample2_bwd(_p(tmp), _p(cur), B, d["h"] // 2, d["w"] // 2, cu, s))
+        for j in range(2):  # bottleneck
+            self._conv_back(k, cur, nxt, B)
+            cur, nxt = nxt, cur
+            k -= 1
+        for i in range(self.levels - 2, -1, -1):  # encoder levels, coarsest first
+            d = self.lay[k]
+            _ck(L.mpd_nn_avgpool2_bwd(_p(cur), _p(nxt), B, d["h"], d["w"], d["co"], s))
+            _ck(L.mpd_nn_add(_p(nxt), _p(self.dskip[i]), B * d["h"] * d["w"] * d["co"], s))
+            cur, nxt = nxt, cur
+            self._conv_back(k, cur, nxt, B)
+            cur, nxt = nxt, cur
+            k -= 1
+            self._conv_back(k, cur, nxt if k > 0 else None, B)
+            cur, nxt = nxt, cur
+            k -= 1
+        assert k == -1
+
+    def adam_step(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+        L, s = nn_lib(), _stream()
+        self.step += 1
+        for k in range(len(self.layers)):
+            _ck(L.mpd_nn_adam(_p(self.w[k]), _p(self.wb[k]), _p(self.gw[k]), _p(self.mw[k]), _p(self.vw[k]),
+                              self.w[k].numel(), lr, b1, b2, eps, self.step, s))
+            _ck(L.mpd_nn_adam(_p(self.b[k]), None, _p(self.gb[k]), _p(self.mb[k]), _p(self.vb[k]),
+                              self.b[k].numel(), lr, b1, b2, eps, self.step, s))
+
+    def mse(self, y, t, dy):
+        """Mean squared error (host float) and dL/dy into dy (may be None)."""
+        L, s = nn_lib(), _stream()
+        n = y.numel()
+        _ck(L.mpd_nn_mse(_p(y), _p(t), _p(dy) if dy is not None else None, _p(self.loss_part), n, s))
+        nb = L.mpd_nn_mse_blocks(n)
+        return float(self.loss_part.download()[:nb].astype(np.float64).sum()) / n
+
+
+# ---------------------------------------------------------------- features
+
+def scale_params(params):
+    """Parameters to [-1, 1] over the sweep ranges."""
+    return 2.0 * (np.asarray(params, np.float64) - PARAM_LO) / (PARAM_HI - PARAM_LO) - 1.0
+
+
+def features(params, grid_r, L_max, M_max):
+    """(N, L_max, M_max+1, 8) float32 input features of N cases: the scaled
+    parameters, 2 z - 1, 2 frac - 1 and (r - 0.5) / 0.3 at every interior
+    cell; grid_r: (N, L_max+2, M_max+3) radii of each case's grid."""
+    N = len(params)
+    Mn = M_max + 1
+    sp = scale_params(params)
+    out = np.empty((N, L_max, Mn, N_FEATURES), np.float32)
+    z = (np.arange(L_max) + 0.0) / L_max
+    frac = np.arange(Mn) / M_max
+    out[..., :5] = sp[:, None, None, :]
+    out[..., 5] = (2 * z - 1)[None, :, None]
+    out[..., 6] = (2 * frac - 1)[None, None, :]
+    out[..., 7] = (np.asarray(grid_r)[:, 1:L_max + 1, 1:Mn + 1] - 0.5) / 0.3
+    return out
+
+
+def baseline_fields(params, grid_r, L_max, M_max, gamma):
+    """(N, L_max, Mn, 8) physical fields of the uniform inflow state each
+    case starts from cold (rho_in, u_z_in, B_phi = amp / r, p = 0.5): the
+    network predicts the departure of the steady state from it."""
+    params = np.asarray(params, np.float64)
+    N = len(params)
+    Mn = M_max + 1
+    r = np.asarray(grid_r)[:, 1:L_max + 1, 1:Mn + 1]
+    out = np.zeros((N, L_max, Mn, 8))
+    out[..., 0] = params[:, 0, None, None]
+    out[..., 1] = params[:, 1, None, None]
+    out[..., 4] = 0.5 / ((gamma - 1.0) * params[:, 0, None, None])
+    out[..., 7] = params[:, 2, None, None] / r
+    return out
+
+
+def fields_from_phys(phys, L_max, M_max):
+    """(L_max, Mn, 8) interior physical fields (OUT_FIELDS order) from the
+    solver's 10 physical arrays."""
+    Mn = M_max + 1
+    return np.stack([phys[PHYS_INDEX[f]][1:L_max + 1, 1:Mn + 1] for f in OUT_FIELDS], axis=-1)
+
+
+def u0_from_fields(f, r, rows, cols):
+    """The conserved arrays u0_1..u0_8 (rows x cols, ghost ring zero) of
+    interior physical fields f (L_max, Mn, 8): RebuildUFromPhysical
+    (include/physics_utils.hpp:15-24)."""
+    L_max, Mn = f.shape[0], f.shape[1]
+    rho, vz, vr, vphi, e, hz, hr, hphi = (f[..., k] for k in range(8))
+    rr = r[1:L_max + 1, 1:Mn + 1]
+    vals = [rho * rr, rho * vz * rr, rho * vr * rr, rho * vphi * rr, rho * e * rr, hphi, hz * rr, hr * rr]
+    out = []
+    for v in vals:
+        a = np.zeros((rows, cols))
+        a[1:L_max + 1, 1:Mn + 1] = v
+        out.append(a)
+    return out

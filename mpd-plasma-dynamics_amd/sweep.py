@@ -1,0 +1,121 @@
+"""Parameter sweeps of the LF solver: config texts of seeded sweep cases and
+the batched runs that take them to a steady state.
+
+A sweep case is the converging-diverging coaxial channel
+r_in(z) = 0.2 + a_in sin(pi z), r_out(z) = 0.8 - a_out z^2 with a fixed
+inflow at z = 0 (rho_in, u_z = u_in, B_phi = amp / r, p = 0.5), zero-gradient
+outflow at z = 1 and slip walls, started from the uniform inflow state (the
+cold start), gamma = 5/3, adaptive CFL 0.4.  Its 5 parameters are
+(rho_in, u_in, amp, a_in, a_out).  The text is config.yaml in the
+reference's schema (include/config.hpp:119-177) with the schema extensions
+documented for this library (time.max_steps, geometry converging_diverging).
+"""
+import time
+
+import numpy as np
+
+PARAM_NAMES = ["rho_in", "u_in", "amp", "a_in", "a_out"]
+PARAM_LO = np.array([0.5, 0.05, 0.05, 0.0, 0.0])
+PARAM_HI = np.array([2.0, 0.3, 0.5, 0.1, 0.1])
+
+
+def sample_params(n, seed, fixed_walls=None):
+    """n seeded uniform draws over the parameter box (SplitMix-free: numpy's
+    PCG64 with the given seed).  fixed_walls=(a_in, a_out) pins the wall
+    amplitudes."""
+    rng = np.random.default_rng(seed)
+    p = PARAM_LO + (PARAM_HI - PARAM_LO) * rng.random((n, 5))
+    if fixed_walls is not None:
+        p[:, 3], p[:, 4] = fixed_walls
+    return p
+
+
+def case_config(params, L_max, M_max, threshold=1e-8, check_frequency=100, max_steps=0, dt=1.0e-4):
+    rho, uz, amp, a_in, a_out = (repr(float(v)) for v in params)
+    return f"""physics:
+  gamma: 1.6666666666666667
+  beta: 0.05
+  H_z0: 0.25
+time:
+  T: 1.0e9
+  dt: {dt}
+  max_steps: {int(max_steps)}
+adaptive_dt:
+  enabled: true
+  cfl_number: 0.4
+  growth_factor: 1.1
+  dt_min: 1.0e-12
+  dt_max: 1.0e-2
+convergence:
+  threshold: {threshold}
+  check_frequency: {int(check_frequency)}
+grid:
+  L_max: {int(L_max)}
+  M_max: {int(M_max)}
+geometry:
+  type: converging_diverging
+  params: {{ r_inner: 0.2, a_inner: {a_in}, r_outer: 0.8, a_outer: {a_out} }}
+initial_conditions:
+  rho: {rho}
+  v_z: {uz}
+  v_r: 0.0
+  v_phi: 0.0
+  H_z: 0.0
+  H_r: 0.0
+  H_phi: "{amp} / r"
+  e: "0.5 / ((gamma - 1) * rho)"
+boundary_conditions:
+  l_lo:
+    - rho: {{ dirichlet: {rho} }}
+      v_z: {{ dirichlet: {uz} }}
+      v_r: {{ dirichlet: 0.0 }}
+      v_phi: {{ dirichlet: 0.0 }}
+      e: "0.5 / ((gamma - 1) * rho)"
+      H_z: {{ dirichlet: 0.0 }}
+      H_r: {{ dirichlet: 0.0 }}
+      H_phi: "{amp} / r"
+  l_hi:
+    - rho: neumann
+      v_z: neumann
+      v_r: neumann
+      v_phi: neumann
+      e: neumann
+      H_z: neumann
+      H_r: neumann
+      H_phi: neumann
+  m_lo:
+    - rho: neumann
+      v_z: neumann
+      v_r: "v_z_nb * r_z_nb"
+      v_phi: neumann
+      e: neumann
+      H_z: neumann
+      H_r: "H_z_nb * r_z_nb"
+      H_phi: neumann
+  m_hi:
+    - rho: neumann
+      v_z: neumann
+      v_r: "v_z * r_z"
+      v_phi: neumann
+      e: neumann
+      H_z: neumann
+      H_r: "H_z * r_z"
+      H_phi: neumann
+"""
+
+
+def run_batch(mpd, configs, warm_u0=None, max_steps=0, autotune=True):
+    """Run the cases of `configs` together to their own ends (convergence,
+    step cap).  warm_u0: optional list (per case, None for a cold start) of 8
+    conserved arrays to start from.  Returns (results, wall seconds, batch):
+    the batch stays open so the caller can read the states."""
+    b = mpd.Batch(configs)
+    if warm_u0 is not None:
+        for i, u in enumerate(warm_u0):
+            if u is not None:
+                b.warm_start(i, u)
+    if autotune:
+        b.autotune(5)
+    t0 = time.perf_counter()
+    res = b.run(max_steps)
+    return res, time.perf_counter() - t0, b

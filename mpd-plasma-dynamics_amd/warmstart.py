@@ -1,0 +1,365 @@
+"""The learned warm-start pipeline: steady states from the batched solver,
+training of the encoder-decoder on the GPU, prediction of initial states, and
+the warm-versus-cold comparison in which the plain LF loop is both the judge
+(each warm start is relaxed to the same convergence threshold as a cold
+start) and the fallback (a case whose residual stalls starts again cold).
+
+Run as a script for the full experiment:
+  python warmstart.py --train 2000 --test 64 --L 96 --M 39 --out DIR
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import mpd_amd  # noqa: E402
+import surrogate as SG  # noqa: E402
+import sweep  # noqa: E402
+
+GAMMA = 1.6666666666666667
+
+
+def progress(msg):
+    print(f"[warmstart {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+def grid_radii(params, L, M):
+    """(N, L+2, M+3) cell radii of each case's grid: Grid::Build's
+    r = (1 - frac) r_in(z) + frac r_out(z), z = (l - 1) / L, frac = (m - 1) / M
+    (src/grid.cpp:24-37), ghost ring included."""
+    params = np.asarray(params, np.float64)
+    z = (np.arange(L + 2) - 1.0) * (1.0 / L)
+    frac = (np.arange(M + 3) - 1.0) * (1.0 / M)
+    r_in = 0.2 + params[:, 3, None] * np.sin(np.pi * z)[None, :]
+    r_out = 0.8 - params[:, 4, None] * (z * z)[None, :]
+    return (1.0 - frac)[None, None, :] * r_in[:, :, None] + frac[None, None, :] * r_out[:, :, None]
+
+
+def solve_batch(configs, warm_u0=None, chunk=0, label="batch"):
+    """Runs a batch to the end of every case's loop; with chunk > 0 in slices
+    of chunk steps, recording when (wall seconds from the start) each case
+    finished.  Returns (results, wall seconds, finish times, batch)."""
+    b = mpd_amd.Batch(configs)
+    if warm_u0 is not None:
+        for i, u in enumerate(warm_u0):
+            if u is not None:
+                b.warm_start(i, u)
+    b.autotune(5)
+    n = len(configs)
+    steps = np.zeros(n, int)
+    finish = np.full(n, np.nan)
+    t0 = time.perf_counter()
+    last = -1e9
+    while True:
+        res = b.run(chunk)
+        el = time.perf_counter() - t0
+        steps += np.array([r["steps"] for r in res])
+        for i, r in enumerate(res):
+            if r["done"] and np.isnan(finish[i]):
+                finish[i] = el
+        ndone = sum(r["done"] for r in res)
+        if chunk and (el - last >= 10.0 or ndone == n):
+            progress(f"{label}: {el:.1f}s, {ndone}/{n} done, max steps {steps.max()}")
+            last = el
+        if ndone == n or not chunk:
+            break
+    for i, r in enumerate(res):
+        r["steps"] = int(steps[i])
+    return res, time.perf_counter() - t0, finish, b
+
+
+def steady_fields(b, n, L, M):
+    out = np.empty((n, L, M + 1, 8))
+    for i in range(n):
+        _, ph = b.instance(i).download(u0=False)
+        out[i] = SG.fields_from_phys(ph, L, M)
+    return out
+
+
+def generate(n, L, M, seed, threshold=1e-8, max_steps=400000, batch=512):
+    """n seeded sweep cases solved cold to the threshold by the batched
+    solver.  Returns (params, fields (n, L, Mn, 8), steps, converged)."""
+    params = sweep.sample_params(n, seed)
+    fields = np.empty((n, L, M + 1, 8))
+    steps = np.zeros(n, int)
+    conv = np.zeros(n, bool)
+    for s0 in range(0, n, batch):
+        idx = list(range(s0, min(n, s0 + batch)))
+        cfgs = [sweep.case_config(params[i], L, M, threshold=threshold, max_steps=max_steps) for i in idx]
+        res, wall, _, b = solve_batch(cfgs, chunk=20000, label=f"generate {s0}-{idx[-1]}")
+        fields[idx] = steady_fields(b, len(idx), L, M)
+        steps[idx] = [r["steps"] for r in res]
+        conv[idx] = [bool(r["converged"]) for r in res]
+        b.close()
+    return params, fields, steps, conv
+
+
+class Normalizer:
+    """Targets: (steady fields - uniform inflow state - mean) / std per
+    channel, statistics over the training set.  A channel without variance
+    in the training set (a field the sweep never excites, e.g. v_phi, H_z,
+    H_r here) is predicted as its mean exactly."""
+
+    def __init__(self, resid, mean=None, std=None):
+        self.mean = resid.reshape(-1, 8).mean(axis=0) if mean is None else np.asarray(mean, np.float64)
+        self.std = resid.reshape(-1, 8).std(axis=0) if std is None else np.asarray(std, np.float64)
+        self.live = self.std > 1e-12 * np.maximum(1.0, np.abs(self.mean))
+
+    def encode(self, resid):
+        return np.where(self.live, (resid - self.mean) / np.where(self.live, self.std, 1.0), 0.0)
+
+    def decode(self, out):
+        return np.where(self.live, out * self.std, 0.0) + self.mean
+
+    def to_json(self):
+        return {"mean": self.mean.tolist(), "std": self.std.tolist()}
+
+
+def train(net, X, T, epochs, batch, lr, seed=0, log_every=20):
+    """Adam on the GPU kernels.  X: (N, H, W, 8) float32 features, T: (N, H,
+    W, 8) float32 normalized targets (host arrays, uploaded once).  Cosine
+    learning-rate decay.  Returns the per-epoch mean losses."""
+    N, H, W, C = X.shape
+    Xd = SG.DeviceArray.from_numpy(X.reshape(N, -1))
+    Td = SG.DeviceArray.from_numpy(T.reshape(N, -1))
+    xb = SG.DeviceArray((batch, H, W, C))
+    xb16 = SG.DeviceArray((batch, H, W, C), "bf16")
+    tb = SG.DeviceArray((batch, H, W, T.shape[-1]))
+    dy = SG.DeviceArray((batch, H, W, T.shape[-1]))
+    idx = SG.DeviceArray(batch, "i32")
+    L = SG.nn_lib()
+    rng = np.random.default_rng(seed)
+    nb = N // batch
+    total = epochs * nb
+    losses = []
+    it = 0
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        perm = rng.permutation(N)
+        acc = 0.0
+        for j in range(nb):
+            idx.upload(perm[j * batch:(j + 1) * batch].astype(np.int32))
+            SG._ck(L.mpd_nn_gather_rows(SG._p(Xd), SG._p(idx), SG._p(xb), batch, H * W * C, None))
+            SG._ck(L.mpd_nn_to_bf16(SG._p(xb), SG._p(xb16), xb.size, None))
+            SG._ck(L.mpd_nn_gather_rows(SG._p(Td), SG._p(idx), SG._p(tb), batch, tb.size // batch, None))
+            y = net.forward(xb16, batch)
+            acc += net.mse(y, tb, dy)
+            net.backward(dy, batch)
+            lr_t = 0.5 * lr * (1 + np.cos(np.pi * it / total))
+            net.adam_step(float(lr_t))
+            it += 1
+        losses.append(acc / nb)
+        if ep % log_every == 0 or ep == epochs - 1:
+            progress(f"epoch {ep}: loss {losses[-1]:.4g} ({time.perf_counter() - t0:.1f}s)")
+    return losses
+
+
+def predict(net, norm, params, radii, L, M, batch=32):
+    """Predicted steady physical fields (n, L, Mn, 8) of the cases."""
+    n = len(params)
+    X = SG.features(params, radii, L, M)
+    base = SG.baseline_fields(params, radii, L, M, GAMMA)
+    out = np.empty((n, L, M + 1, 8))
+    xb16 = SG.DeviceArray((batch, L, M + 1, SG.N_FEATURES), "bf16")
+    for s0 in range(0, n, batch):
+        k = min(batch, n - s0)
+        xx = np.zeros((batch,) + X.shape[1:], np.float32)
+        xx[:k] = X[s0:s0 + k]
+        xb16.upload(xx)
+        y = net.forward(xb16, batch).download()
+        out[s0:s0 + k] = base[s0:s0 + k] + norm.decode(y[:k].astype(np.float64))
+    return out
+
+
+def warm_u0(fields, radii, L, M):
+    rows, cols = L + 2, M + 3
+    return [SG.u0_from_fields(fields[i], radii[i], rows, cols) for i in range(len(fields))]
+
+
+def field_errors(a, b):
+    """Per-field relative L-inf error max|a - b| / max|b| (OUT_FIELDS order;
+    fields that are identically zero in b report the absolute error)."""
+    errs = []
+    for k in range(8):
+        scale = np.max(np.abs(b[..., k]))
+        d = np.max(np.abs(a[..., k] - b[..., k]))
+        errs.append(float(d / scale) if scale > 0 else float(d))
+    return errs
+
+
+def admissible(fields):
+    """A predicted state the solver may start from: finite everywhere, with
+    positive density and specific internal energy (so positive pressure)."""
+    f = np.asarray(fields)
+    return bool(np.all(np.isfinite(f)) and np.all(f[..., 0] > 0) and np.all(f[..., 4] > 0))
+
+
+def stalled(done, converged, change, change_mid, stall_change):
+    """Fallback rule after the stall check's steps: a case that ended without
+    converging (a non-finite state ends the loop once t becomes NaN), or that
+    still runs with a non-finite SolutionChange, or whose SolutionChange is
+    above stall_change and did not fall over the second half of the check's
+    steps."""
+    if done:
+        return not converged
+    if not np.isfinite(change):
+        return True
+    return bool(change > stall_change and not (change < change_mid))
+
+
+def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_change=1e-6, plain_sample=None,
+             chunk=1000, fallback=True, pred=None):
+    """Cold-started plain solves (each case alone, mpd_run) against the
+    surrogate-warm-started batched sweep, both to the same threshold.  A
+    warm case whose SolutionChange is still above stall_change after
+    stall_steps steps is restarted cold (fallback).  plain_sample: time only
+    that many plain solves and scale to all cases (stated in the result)."""
+    n = len(params)
+    cfgs = [sweep.case_config(p, L, M, threshold=threshold, max_steps=2000000) for p in params]
+    radii = grid_radii(params, L, M)
+
+    # Plain cold solves, one case at a time (the reference's way: one run per case).
+    m = n if plain_sample is None else min(n, plain_sample)
+    plain_steps, plain_t = [], []
+    for i in range(m):
+        with mpd_amd.Solver(cfgs[i]) as s:
+            t0 = time.perf_counter()
+            r = s.run()
+            plain_t.append(time.perf_counter() - t0)
+            plain_steps.append(r["steps"])
+    plain_total = float(np.sum(plain_t)) * (n / m)
+    progress(f"plain cold solves: {m} timed, {np.mean(plain_t):.3f}s mean, {plain_total:.2f}s for {n}")
+
+    # Cold batched (the batching gain alone) and its steady states.
+    cold_res, cold_wall, cold_finish, bc = solve_batch(cfgs, chunk=chunk, label="cold batch")
+    cold_fields = steady_fields(bc, n, L, M)
+    bc.close()
+    progress(f"cold batch: {cold_wall:.2f}s")
+
+    # Warm batched: inference, warm start, stall check, fallback.
+    t0 = time.perf_counter()
+    if pred is None:
+        pred = predict(net, norm, params, radii, L, M)
+    infer = time.perf_counter() - t0
+    b = mpd_amd.Batch(cfgs)
+    rejected = [i for i in range(n) if not admissible(pred[i])]
+    for i, u in enumerate(warm_u0(pred, radii, L, M)):
+        if i not in rejected:
+            b.warm_start(i, u)
+    b.autotune(5)
+    t1 = time.perf_counter()
+    # Stall check: after stall_steps, a case still running with its
+    # SolutionChange above stall_change that did not decrease over the second
+    # half of those steps (or is not finite) starts again cold.
+    res = b.run(stall_steps // 2)
+    steps = np.array([r["steps"] for r in res])
+    mid = np.array([r["change"] for r in res])
+    res = b.run(stall_steps - stall_steps // 2)
+    steps += np.array([r["steps"] for r in res])
+    finish = np.full(n, np.nan)
+    el = time.perf_counter() - t1
+    for i, r in enumerate(res):
+        if r["done"]:
+            finish[i] = el
+    stall = [i for i, r in enumerate(res) if fallback and i not in rejected and
+             stalled(r["done"], r["converged"], r["change"], mid[i], stall_change)]
+    for i in stall:
+        b.instance(i).cold_restart(cfgs[i])
+        finish[i] = np.nan
+    while True:
+        res = b.run(chunk)
+        el = time.perf_counter() - t1
+        steps += np.array([r["steps"] for r in res])
+        for i, r in enumerate(res):
+            if r["done"] and np.isnan(finish[i]):
+                finish[i] = el
+        if all(r["done"] for r in res):
+            break
+    warm_wall = time.perf_counter() - t1
+    warm_fields = steady_fields(b, n, L, M)
+    b.close()
+    progress(f"warm batch: inference {infer:.3f}s, relax {warm_wall:.2f}s, {len(rejected)} rejected, "
+             f"{len(stall)} stalled")
+
+    errs = np.array([field_errors(warm_fields[i], cold_fields[i]) for i in range(n)])
+    pred_errs = np.array([field_errors(pred[i], cold_fields[i]) for i in range(n)])
+    cold_steps = np.array([r["steps"] for r in cold_res])
+    return {
+        "cases": n, "grid": [L, M], "threshold": threshold,
+        "plain_cold_total_s": plain_total, "plain_timed_cases": m,
+        "plain_cold_steps_mean": float(np.mean(plain_steps)),
+        "cold_batch_s": cold_wall, "cold_steps_p50": float(np.median(cold_steps)),
+        "cold_steps_max": int(cold_steps.max()),
+        "warm_inference_s": infer, "warm_batch_s": warm_wall, "warm_total_s": infer + warm_wall,
+        "warm_steps_p50": float(np.median(steps)), "warm_steps_max": int(steps.max()),
+        "steps_saved_mean": float(np.mean(cold_steps - steps)),
+        "steps_ratio_cold_over_warm": float(np.sum(cold_steps) / np.sum(steps)),
+        "fallbacks": len(rejected) + len(stall), "rejected_predictions": len(rejected),
+        "stalled": len(stall),
+        "all_converged_cold": bool(all(r["converged"] for r in cold_res)),
+        "all_converged_warm": bool(all(r["converged"] for r in res)),
+        "speedup_vs_plain_cold": plain_total / (infer + warm_wall),
+        "speedup_vs_cold_batch": cold_wall / (infer + warm_wall),
+        "warm_finish_p50_s": float(np.nanmedian(finish)), "warm_finish_p99_s": float(np.nanpercentile(finish, 99)),
+        "cold_finish_p50_s": float(np.nanmedian(cold_finish)),
+        "cold_finish_p99_s": float(np.nanpercentile(cold_finish, 99)),
+        "max_field_err_warm_vs_cold": dict(zip(SG.OUT_FIELDS, errs.max(axis=0).tolist())),
+        "max_field_err_prediction_vs_cold": dict(zip(SG.OUT_FIELDS, pred_errs.max(axis=0).tolist())),
+        "median_field_err_prediction_vs_cold": dict(zip(SG.OUT_FIELDS, np.median(pred_errs, axis=0).tolist())),
+    }, warm_fields, cold_fields
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--test", type=int, default=64)
+    ap.add_argument("--L", type=int, default=96)
+    ap.add_argument("--M", type=int, default=39)
+    ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--out", default="")
+    args = ap.parse_args()
+    L, M = args.L, args.M
+    t0 = time.perf_counter()
+    params, fields, steps, conv = generate(args.train, L, M, args.seed)
+    gen_s = time.perf_counter() - t0
+    keep = conv
+    progress(f"generated {args.train} steady states in {gen_s:.1f}s ({keep.sum()} converged, steps p50 "
+             f"{np.median(steps):.0f} max {steps.max()})")
+    params, fields = params[keep], fields[keep]
+    radii = grid_radii(params, L, M)
+    base = SG.baseline_fields(params, radii, L, M, GAMMA)
+    norm = Normalizer(fields - base)
+    X = SG.features(params, radii, L, M)
+    T = norm.encode(fields - base).astype(np.float32)
+    net = SG.UNet(L, M + 1, seed=args.seed)
+    progress(f"network: {net.n_params()} parameters, {len(net.layers)} convs")
+    t1 = time.perf_counter()
+    losses = train(net, X, T, args.epochs, args.batch, args.lr, seed=args.seed)
+    train_s = time.perf_counter() - t1
+    test_params = sweep.sample_params(args.test, args.seed + 1)
+    result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=args.plain_sample)
+    result.update({"train_cases": int(keep.sum()), "generate_s": gen_s, "train_s": train_s, "epochs": args.epochs,
+                   "final_loss": losses[-1], "first_loss": losses[0], "n_params": net.n_params()})
+    print(json.dumps(result, indent=1), flush=True)
+    if args.out:
+        os.makedirs(args.out, exist_ok=True)
+        with open(os.path.join(args.out, "warmstart_result.json"), "w") as f:
+            json.dump(result, f, indent=1)
+        sd = net.state_dict()
+        np.savez(os.path.join(args.out, "warmstart_net.npz"), **{f"w{k}": w for k, w in enumerate(sd["w"])},
+                 **{f"b{k}": b for k, b in enumerate(sd["b"])}, mean=norm.mean, std=norm.std)
+
+
+if __name__ == "__main__":
+    main()

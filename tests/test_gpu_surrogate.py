@@ -1,0 +1,222 @@
+"""GPU tests of the warm-start network's kernels (libmpd_nn.so) against the
+float64 oracle (oracle/surrogate_oracle.py) fed the same bf16 operands.
+
+Tolerances: a convolution accumulates exact bf16 x bf16 products in FP32, so
+it matches the float64 result to a few FP32 ulps of the accumulated
+magnitude (bound written in each test).  Where the kernels round an FP32
+value to bf16 (SiLU output, pooled values), an FP32-level difference can move
+the rounding by one bf16 ulp (2^-8 relative): those comparisons allow one
+ulp.  Whole-network gradients pass through many such roundings and are
+compared in norm."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+import surrogate_oracle as S  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def nn():
+    import surrogate
+    surrogate.nn_lib()
+    return surrogate
+
+
+def _dev_bf16(a):
+    import surrogate
+    return surrogate.DeviceArray.from_numpy(a, "bf16")
+
+
+def _dev_f32(a):
+    import surrogate
+    return surrogate.DeviceArray.from_numpy(a, "f32")
+
+
+def _host(t):
+    return t.download().astype(np.float64)
+
+
+def _host_bf16(t):
+    return t.download().astype(np.float64)
+
+
+def _empty(shape, bf16=False):
+    import surrogate
+    return surrogate.DeviceArray(shape, "bf16" if bf16 else "f32")
+
+
+SHAPES = [(2, 8, 5, 3, 5), (3, 12, 10, 8, 32), (1, 16, 8, 96, 32), (2, 6, 5, 40, 70), (1, 5, 3, 256, 130)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_conv_passes_match_oracle(nn, shape):
+    B, H, W, Ci, Co = shape
+    print(f"conv shape B={B} H={H} W={W} Cin={Ci} Cout={Co}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    x = S.bf16_round(rng.normal(size=(B, H, W, Ci)))
+    w = S.bf16_round(rng.normal(0, 0.3, size=(Co, 3, 3, Ci)))
+    b = rng.normal(size=Co).astype(np.float32).astype(np.float64)
+    dz = S.bf16_round(rng.normal(size=(B, H, W, Co)))
+    L, s = nn.nn_lib(), nn._stream()
+    X, Wt, Bt, DZ = _dev_bf16(x), _dev_bf16(w), _dev_f32(b), _dev_bf16(dz)
+    Z = _empty((B, H, W, Co))
+    nn._ck(L.mpd_nn_conv3x3_fwd(nn._p(X), nn._p(Wt), nn._p(Bt), nn._p(Z), B, H, W, Ci, Co, s))
+    DX = _empty((B, H, W, Ci))
+    nn._ck(L.mpd_nn_conv3x3_bwd_data(nn._p(DZ), nn._p(Wt), nn._p(DX), B, H, W, Ci, Co, s))
+    work = _empty((int(L.mpd_nn_wgrad_work_floats(B, H, W, Ci, Co)),))
+    DW = _empty((Co, 3, 3, Ci))
+    nn._ck(L.mpd_nn_conv3x3_bwd_weight(nn._p(X), nn._p(DZ), nn._p(DW), nn._p(work), B, H, W, Ci, Co, s))
+    # Bound: FP32 accumulation error <= K * 2^-24 * sum |terms| (K terms).
+    ref = S.conv_fwd(x, w, b)
+    mag = S.conv_fwd(np.abs(x), np.abs(w)) + np.abs(b)
+    assert np.all(np.abs(_host(Z) - ref) <= 9 * Ci * 2.0 ** -23 * mag + 1e-30)
+    ref = S.conv_bwd_data(dz, w, x.shape)
+    mag = S.conv_bwd_data(np.abs(dz), np.abs(w), x.shape)
+    assert np.all(np.abs(_host(DX) - ref) <= 9 * Co * 2.0 ** -23 * mag + 1e-30)
+    ref = S.conv_bwd_weight(x, dz)
+    mag = S.conv_bwd_weight(np.abs(x), np.abs(dz))
+    assert np.all(np.abs(_host(DW) - ref) <= B * H * W * 2.0 ** -23 * mag + 1e-30)
+
+
+def test_elementwise_kernels_match_oracle(nn):
+    L, s = nn.nn_lib(), nn._stream()
+    rng = np.random.default_rng(7)
+    B, H, W, C = 2, 6, 8, 5
+    z = rng.normal(0, 2, size=(B, H, W, C)).astype(np.float32).astype(np.float64)
+    n = z.size
+    Z = _dev_f32(z)
+    A = _empty(z.shape, bf16=True)
+    nn._ck(L.mpd_nn_silu_fwd(nn._p(Z), nn._p(A), n, s))
+    ref = S.silu(z)
+    assert np.all(np.abs(_host_bf16(A) - ref) <= 2.0 ** -7 * np.abs(ref) + 1e-30)
+    da = rng.normal(size=z.shape).astype(np.float32).astype(np.float64)
+    DZB, DZF = _empty(z.shape, bf16=True), _empty(z.shape)
+    nn._ck(L.mpd_nn_silu_bwd(nn._p(Z), nn._p(_dev_f32(da)), nn._p(DZB), nn._p(DZF), n, s))
+    ref = da * S.silu_grad(z)
+    np.testing.assert_allclose(_host(DZF), ref, rtol=2e-6, atol=1e-6)
+    assert np.all(np.abs(_host_bf16(DZB) - ref) <= 2.0 ** -7 * np.abs(ref) + 1e-6)
+
+    x = S.bf16_round(rng.normal(size=(B, H, W, C)))
+    X = _dev_bf16(x)
+    P = _empty((B, H // 2, W // 2, C), bf16=True)
+    nn._ck(L.mpd_nn_avgpool2_fwd(nn._p(X), nn._p(P), B, H, W, C, s))
+    assert np.array_equal(_host_bf16(P), S.bf16_round(S.pool_fwd(x)))
+    dy = rng.normal(size=(B, H // 2, W // 2, C)).astype(np.float32).astype(np.float64)
+    DX = _empty((B, H, W, C))
+    nn._ck(L.mpd_nn_avgpool2_bwd(nn._p(_dev_f32(dy)), nn._p(DX), B, H, W, C, s))
+    assert np.array_equal(_host(DX), S.pool_bwd(dy))
+    U = _empty((B, 2 * H, 2 * W, C), bf16=True)
+    nn._ck(L.mpd_nn_upsample2_fwd(nn._p(X), nn._p(U), B, H, W, C, s))
+    assert np.array_equal(_host_bf16(U), S.up_fwd(x))
+    du = rng.normal(size=(B, 2 * H, 2 * W, C)).astype(np.float32).astype(np.float64)
+    DU = _empty((B, H, W, C))
+    nn._ck(L.mpd_nn_upsample2_bwd(nn._p(_dev_f32(du)), nn._p(DU), B, H, W, C, s))
+    np.testing.assert_allclose(_host(DU), S.up_bwd(du), rtol=1e-6, atol=1e-6)
+
+    y2 = S.bf16_round(rng.normal(size=(B, H, W, 3)))
+    CT = _empty((B, H, W, C + 3), bf16=True)
+    nn._ck(L.mpd_nn_concat_fwd(nn._p(X), C, nn._p(_dev_bf16(y2)), 3, nn._p(CT), B * H * W, s))
+    assert np.array_equal(_host_bf16(CT), np.concatenate([x, y2], axis=-1))
+    dc = rng.normal(size=(B, H, W, C + 3)).astype(np.float32).astype(np.float64)
+    base_b = rng.normal(size=(B, H, W, 3)).astype(np.float32).astype(np.float64)
+    DA, DB = _empty((B, H, W, C)), _dev_f32(base_b)
+    nn._ck(L.mpd_nn_concat_bwd(nn._p(_dev_f32(dc)), nn._p(DA), C, 0, nn._p(DB), 3, 1, B * H * W, s))
+    assert np.array_equal(_host(DA), dc[..., :C])
+    np.testing.assert_allclose(_host(DB), base_b + dc[..., C:], rtol=1e-6)
+
+    t = rng.normal(size=z.shape).astype(np.float32).astype(np.float64)
+    DY = _empty(z.shape)
+    part = _empty((1024,))
+    nn._ck(L.mpd_nn_mse(nn._p(Z), nn._p(_dev_f32(t)), nn._p(DY), nn._p(part), n, s))
+    loss = _host(part)[:L.mpd_nn_mse_blocks(n)].sum() / n
+    rl, rdy = S.mse(z, t)
+    assert abs(loss - rl) <= 1e-5 * rl
+    np.testing.assert_allclose(_host(DY), rdy, rtol=1e-5, atol=1e-9)
+
+    w0 = rng.normal(size=50).astype(np.float32).astype(np.float64)
+    g = rng.normal(size=50).astype(np.float32).astype(np.float64)
+    Wd, G, M, V = _dev_f32(w0), _dev_f32(g), _empty((50,)), _empty((50,))
+    M.zero()
+    V.zero()
+    WB = _empty((50,), bf16=True)
+    for step in (1, 2):
+        nn._ck(L.mpd_nn_adam(nn._p(Wd), nn._p(WB), nn._p(G), nn._p(M), nn._p(V), 50, 1e-2, 0.9, 0.999, 1e-8, step, s))
+    rw, rm, rv = w0, np.zeros(50), np.zeros(50)
+    for step in (1, 2):
+        rw, rm, rv = S.adam(rw, g, rm, rv, 1e-2, 0.9, 0.999, 1e-8, step)
+    np.testing.assert_allclose(_host(Wd), rw, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(_host_bf16(WB), S.bf16_round(_host(Wd)))
+
+    src = rng.normal(size=(6, 4)).astype(np.float32)
+    idx = np.array([5, 0, 3], np.int32)
+    dst = _empty((3, 4))
+    I = nn.DeviceArray.from_numpy(idx, "i32")
+    nn._ck(L.mpd_nn_gather_rows(nn._p(_dev_f32(src)), nn._p(I), nn._p(dst), 3, 4, s))
+    assert np.array_equal(_host(dst), src[idx].astype(np.float64))
+
+
+@pytest.mark.parametrize("base,H,W,B", [(8, 16, 8, 2), (32, 16, 8, 2), (32, 24, 40, 1)])
+def test_unet_forward_backward_match_oracle(nn, base, H, W, B):
+    """The whole encoder-decoder (4 levels) on the GPU against the oracle with
+    the same bf16 roundings: the loss, and every parameter gradient (in
+    norm: 1e-2 relative, cosine > 0.9999)."""
+    print(f"unet base={base} grid={H}x{W} batch={B}", flush=True)
+    rng = np.random.default_rng(base + H)
+    net = nn.UNet(H, W, base=base, seed=3)
+    ws = [rng.normal(0, np.sqrt(1.0 / (9 * ci)), size=(co, 3, 3, ci)) for ci, co in net.layers]
+    bs = [rng.normal(0, 0.05, size=co) for _, co in net.layers]
+    net.load_params(ws, bs)
+    x = S.bf16_round(rng.normal(size=(B, H, W, 8)))
+    t = rng.normal(size=(B, H, W, 8)).astype(np.float32).astype(np.float64)
+    y = net.forward(_dev_bf16(x), B)
+    T = _dev_f32(t)
+    DY = _empty((B, H, W, 8))
+    loss = net.mse(y, T, DY)
+    net.backward(DY, B)
+    params = [(S.bf16_round(np.asarray(w, np.float32)), np.asarray(b, np.float32).astype(np.float64))
+              for w, b in zip(ws, bs)]
+    ry, tape = S.unet_forward(x, params, bf16=True)
+    rl, rdy = S.mse(ry, t)
+    assert abs(loss - rl) <= 1e-3 * rl, (loss, rl)
+    yg = _host(y)
+    assert np.linalg.norm(yg - ry) <= 2e-3 * np.linalg.norm(ry)
+    grads = S.unet_backward(rdy, tape, bf16=True)
+    for k, (gw, gb) in enumerate(grads):
+        for got, ref in ((_host(net.gw[k]), gw), (_host(net.gb[k]), gb)):
+            rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            cos = np.sum(got * ref) / (np.linalg.norm(got) * np.linalg.norm(ref))
+            assert rel < 1e-2 and cos > 0.9999, (k, rel, cos)
+
+
+def test_training_fits_a_smooth_field(nn):
+    """Adam on the GPU kernels fits a smooth parameter-dependent field: the
+    loss falls by more than 20x in 150 steps."""
+    H, W, B = 16, 8, 16
+    rng = np.random.default_rng(11)
+    p = rng.uniform(-1, 1, size=(B, 3))
+    yy, xx = np.meshgrid(np.linspace(-1, 1, H), np.linspace(-1, 1, W), indexing="ij")
+    x = np.zeros((B, H, W, 8))
+    x[..., :3] = p[:, None, None, :]
+    x[..., 3] = yy
+    x[..., 4] = xx
+    t = np.zeros((B, H, W, 8))
+    for c in range(8):
+        t[..., c] = np.sin((c + 1) * 0.5 * yy + p[:, 0, None, None]) * (1 + 0.5 * p[:, 1, None, None] * xx)
+    net = nn.UNet(H, W, base=16, seed=1)
+    X, T = _dev_bf16(x), _dev_f32(t)
+    DY = _empty((B, H, W, 8))
+    losses = []
+    for it in range(150):
+        y = net.forward(X, B)
+        losses.append(net.mse(y, T, DY))
+        net.backward(DY, B)
+        net.adam_step(2e-3)
+    print(f"loss {losses[0]:.4g} -> {losses[-1]:.4g}", flush=True)
+    assert losses[-1] < losses[0] / 20
+    assert np.all(np.isfinite(losses))

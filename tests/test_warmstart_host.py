@@ -1,0 +1,91 @@
+"""Host-side (CPU) tests of the warm-start pipeline's pure logic: the
+fallback rule, the admissibility screen of predictions, the target
+normalization, the features and the conserved-state rebuild."""
+import numpy as np
+
+import surrogate as SG
+import sweep
+import warmstart as WS
+
+
+def test_stall_rule():
+    # Ended without converging (e.g. t became NaN): fall back.
+    assert WS.stalled(True, False, 1e-3, 1e-3, 1e-6)
+    assert not WS.stalled(True, True, 5e-9, 1e-7, 1e-6)
+    # Running: non-finite change, or above the threshold and not falling.
+    assert WS.stalled(False, False, float("nan"), 1e-4, 1e-6)
+    assert WS.stalled(False, False, 2e-4, 1e-4, 1e-6)
+    assert WS.stalled(False, False, 1e-4, 1e-4, 1e-6)
+    assert not WS.stalled(False, False, 0.9e-4, 1e-4, 1e-6)
+    assert not WS.stalled(False, False, 5e-7, 1e-7, 1e-6)
+
+
+def test_admissible():
+    f = np.ones((4, 3, 8))
+    assert WS.admissible(f)
+    g = f.copy(); g[1, 1, 0] = -1e-3
+    assert not WS.admissible(g)
+    g = f.copy(); g[0, 2, 4] = 0.0
+    assert not WS.admissible(g)
+    g = f.copy(); g[3, 0, 6] = np.inf
+    assert not WS.admissible(g)
+    g = f.copy(); g[..., 2] = -5.0  # velocities may be negative
+    assert WS.admissible(g)
+
+
+def test_normalizer_round_trip_and_dead_channels():
+    rng = np.random.default_rng(0)
+    r = rng.normal(size=(10, 6, 5, 8))
+    r[..., 3] = 0.0  # a field the sweep never excites
+    n = WS.Normalizer(r)
+    assert not n.live[3] and n.live[[0, 1, 2, 4, 5, 6, 7]].all()
+    e = n.encode(r)
+    assert np.all(e[..., 3] == 0.0)
+    np.testing.assert_allclose(n.decode(e), r, rtol=1e-12, atol=1e-12)
+    # Whatever the network outputs in a dead channel, the prediction is exact.
+    out = e.copy(); out[..., 3] = 7.0
+    assert np.all(n.decode(out)[..., 3] == 0.0)
+
+
+def test_grid_radii_closed_form():
+    p = np.array([[1.0, 0.1, 0.2, 0.05, 0.1]])
+    r = WS.grid_radii(p, 10, 4)
+    assert r.shape == (1, 12, 7)
+    # Interior cell (l=1, m=1): z = 0, frac = 0 -> r_in(0) = 0.2.
+    assert r[0, 1, 1] == 0.2
+    # (l = L, m = M + 1): z = 0.9, frac = 1 -> r_out(0.9) = 0.8 - 0.1 * 0.81.
+    assert abs(r[0, 10, 5] - (0.8 - 0.1 * 0.81)) < 1e-15
+
+
+def test_u0_from_fields_is_rebuild_from_physical():
+    """RebuildUFromPhysical (include/physics_utils.hpp:15-24): u1 = rho r,
+    u2..u5 = rho (v_z, v_r, v_phi, e) r, u6 = H_phi, u7, u8 = (H_z, H_r) r."""
+    rng = np.random.default_rng(1)
+    L, M = 6, 3
+    f = rng.uniform(0.5, 2.0, size=(L, M + 1, 8))
+    r = rng.uniform(0.2, 0.8, size=(L + 2, M + 3))
+    u = SG.u0_from_fields(f, r, L + 2, M + 3)
+    rr = r[1:L + 1, 1:M + 2]
+    rho, vz, vr, vphi, e, hz, hr, hphi = (f[..., k] for k in range(8))
+    for k, v in enumerate([rho * rr, rho * vz * rr, rho * vr * rr, rho * vphi * rr, rho * e * rr, hphi,
+                           hz * rr, hr * rr]):
+        assert np.array_equal(u[k][1:L + 1, 1:M + 2], v)
+        assert np.all(u[k][0] == 0) and np.all(u[k][:, 0] == 0) and np.all(u[k][L + 1] == 0)
+
+
+def test_features_and_baseline():
+    p = sweep.sample_params(3, 4)
+    r = WS.grid_radii(p, 8, 3)
+    x = SG.features(p, r, 8, 3)
+    assert x.shape == (3, 8, 4, 8) and x.dtype == np.float32
+    assert np.all(np.abs(x[..., :5]) <= 1.0)
+    b = SG.baseline_fields(p, r, 8, 3, WS.GAMMA)
+    np.testing.assert_allclose(b[..., 7], p[:, 2, None, None] / r[:, 1:9, 1:5])
+    np.testing.assert_allclose(b[..., 4] * (WS.GAMMA - 1) * b[..., 0], 0.5)
+
+
+def test_case_config_parses():
+    import mpd_amd
+    p = sweep.sample_params(2, 9)
+    for q in p:
+        assert mpd_amd.config_check(sweep.case_config(q, 20, 8)) == (20, 8)
