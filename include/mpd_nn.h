@@ -49,8 +49,10 @@ int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, in
 long long mpd_nn_wgrad_work_floats(int B, int H, int W, int Cin, int Cout);
 int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, float* work, int B, int H, int W,
                               int Cin, int Cout, void* stream);
-/* db[c] = sum over the P pixels of dz[p][c] (FP32). */
-int mpd_nn_bias_grad(const float* dz, float* db, int P, int C, void* stream);
+/* db[c] = sum over the P pixels of dz[p][c] (FP32), summed in a fixed order
+ * over pixel slices; `work` holds mpd_nn_bias_work_floats(P, C) floats. */
+long long mpd_nn_bias_work_floats(int P, int C);
+int mpd_nn_bias_grad(const float* dz, float* db, float* work, int P, int C, void* stream);
 
 /* a = bf16(silu(z)), silu(z) = z / (1 + exp(-z)); backward: dz = da silu'(z)
  * written as bf16 and (if dz_f32 != NULL) FP32. */

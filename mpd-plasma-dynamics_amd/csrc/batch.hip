@@ -62,6 +62,8 @@ struct mpd_batch {
     // residual entries (in this order in d_work).
     std::vector<char> graph_active;
     int n_st = 0, n_bc = 0, n_res = 0;
+    int bc_words = 0;  // largest BC program image of the work lists' instances
+    int bc_area = 0;   // largest boundary work area (slots per lane)
     hipGraphExec_t graph = nullptr;
     int graph_K = 0;
 
@@ -190,6 +192,12 @@ struct mpd_batch {
             for (int b = 0; b < nb; ++b) out.push_back({i, b, nb, 0, 0});
         }
         nbc = (int)out.size() - nst;
+        bc_words = 0;
+        bc_area = 0;
+        for (int i : order) {
+            bc_words = std::max(bc_words, inst[i]->bc_words);
+            bc_area = std::max(bc_area, inst[i]->bc_area);
+        }
         for (int i : order) {
             if (inst[i]->cfg.convergence_threshold <= 0.0) continue;
             const int nb = ResBlocks(*inst[i]);
@@ -213,7 +221,7 @@ struct mpd_batch {
 
     void EnqueueStep(int flip) {
         HIPCHK(mpd::launch_stencil_batch(d_args, d_icur, d_work, n_st, grid, flip, fast_div, buf, 0.0, 1, stream));
-        HIPCHK(mpd::launch_bc_batch(d_args, d_icur, d_work + n_st, n_bc, flip, stream));
+        HIPCHK(mpd::launch_bc_batch(d_args, d_icur, d_work + n_st, n_bc, flip, bc_words, bc_area, stream));
         if (n_res > 0) HIPCHK(mpd::launch_residual_batch(d_args, d_icur, d_aux, d_work + n_st + n_bc, n_res, flip,
                                                          stream));
     }

@@ -31,6 +31,8 @@ struct StepArgs {
     double cfl_dx, growth, dt_min, dt_max, T;
     int adaptive;
     int step1;            // start-of-step physical comes from phys_in
+    int bc_words;         // size of the BC program's LDS image in 8-byte words (bc_image_words)
+    int bc_area;          // work-area slots per lane of the boundary kernel (bc_area_slots)
     MpdLoopState* st;
     const MpdBcProgram* bc;
 };
@@ -56,6 +58,22 @@ struct BatchAux {
     int check_frequency;
     int pad;
 };
+
+// The LDS image of a BC program (the boundary kernels' BcImage): a 40-byte
+// header, then the segments and the expression code in use.
+constexpr int kBcImageHeadWords = 5;
+__host__ __device__ inline int bc_image_words(const MpdBcProgram& p) {
+    return kBcImageHeadWords + (int)(sizeof(MpdSeg) / 8) * (p.nseg[0] + p.nseg[1] + p.nseg[2] + p.nseg[3]) +
+           2 * p.ncode;
+}
+
+// Per-lane work area of the boundary kernel, in slots of one double: the
+// expression symbol table, the chain state and its neighbour state, then the
+// evaluation stack, as deep as the program's deepest expression needs.
+enum { WA_SYM = 0, WA_CUR = WA_SYM + MPD_BS_COUNT, WA_NBR = WA_CUR + 10, WA_STK = WA_NBR + 10 };
+__host__ __device__ inline int bc_area_slots(const MpdBcProgram& p) {
+    return WA_STK + (p.max_stack > 1 ? p.max_stack : 1);
+}
 
 // Physical fields in the member order of include/fields.hpp:36-41.
 enum { PH_RHO = 0, PH_VZ, PH_VR, PH_VPHI, PH_E, PH_P, PH_PT, PH_HZ, PH_HR, PH_HPHI };

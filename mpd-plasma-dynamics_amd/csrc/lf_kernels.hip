@@ -462,10 +462,8 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled_batch(const StepArgs*
 // Laid out [slot][lane] for the 64-thread workgroups of lf_bc_ring, so one
 // slot of a wave is 64 consecutive doubles.
 #define BC_THREADS 64
-enum { WA_SYM = 0, WA_STK = WA_SYM + MPD_BS_COUNT, WA_CUR = WA_STK + MPD_MAX_STACK, WA_NBR = WA_CUR + 10,
-       WA_N = WA_NBR + 10 };
 struct WorkArea {
-    double* base;  // LDS: WA_N * BC_THREADS doubles
+    double* base;  // LDS: bc_area_slots * BC_THREADS doubles (slot layout WA_* in lf_device.h)
     int lane;
     __device__ __forceinline__ double& at(int i) const { return base[i * BC_THREADS + lane]; }
     __device__ __forceinline__ double& sym(int i) const { return at(WA_SYM + i); }
@@ -482,6 +480,29 @@ struct WorkArea {
         for (int k = 0; k < 10; ++k) v[k] = at(area + k);
     }
 };
+
+// LDS image of a BC program, staged once per workgroup: the counts, then the
+// segments in use (faces in the order L_LO, L_HI, M_LO, M_HI, each face's in
+// config order), then the expression code in use.  Its size follows the
+// program (StepArgs::bc_words), so the boundary kernels take it, with the
+// work area behind it, in dynamic LDS: a sweep's small programs leave room
+// for more workgroups per CU than the largest program would.
+struct BcImage {
+    int32_t nseg[4];
+    int32_t ncode;
+    int32_t first[4];  // index of face f's first segment
+    int32_t nsegs;
+    __device__ __forceinline__ const MpdSeg& seg(int f, int s) const {
+        return reinterpret_cast<const MpdSeg*>(this + 1)[first[f] + s];
+    }
+    __device__ __forceinline__ const MpdInstr& code(int i) const {
+        return reinterpret_cast<const MpdInstr*>(reinterpret_cast<const MpdSeg*>(this + 1) + nsegs)[i];
+    }
+};
+static_assert(sizeof(BcImage) == 8 * kBcImageHeadWords, "image header layout");
+
+// The staged image a boundary kernel's StepArgs points to.
+__device__ __forceinline__ const BcImage* img(const StepArgs& a) { return reinterpret_cast<const BcImage*>(a.bc); }
 
 // The rarely used operations of the expression code, out of line so that
 // the interpreter stays small where it is inlined.
@@ -508,10 +529,10 @@ __device__ __noinline__ double bc_fn2(int op, double x, double y) {
 
 // Evaluate one compiled expression (the exprtk call sites of
 // src/bcs/per_field_bc.cpp:323-358) on the symbol values in the work area.
-__device__ __forceinline__ double eval_expr(const MpdBcProgram* bc, int off, int len, const WorkArea& w) {
+__device__ __forceinline__ double eval_expr(const BcImage* bc, int off, int len, const WorkArea& w) {
     int sp = 0;
     for (int i = 0; i < len; ++i) {
-        const MpdInstr in = bc->code[off + i];
+        const MpdInstr in = bc->code(off + i);
         switch (in.op) {
             case MPD_OP_CONST: w.stk(sp++) = in.val; break;
             case MPD_OP_SYM: w.stk(sp++) = w.sym(in.arg); break;
@@ -574,9 +595,9 @@ __device__ __forceinline__ constexpr int nb_slot(int k) {
 
 __device__ __forceinline__ bool seg_covers(const MpdSeg& s, int i) { return i >= s.lo && i <= s.hi; }
 
-__device__ __forceinline__ bool face_covers(const MpdBcProgram* bc, int f, int i) {
+__device__ __forceinline__ bool face_covers(const BcImage* bc, int f, int i) {
     for (int s = 0; s < bc->nseg[f]; ++s)
-        if (seg_covers(bc->seg[f][s], i)) return true;
+        if (seg_covers(bc->seg(f, s), i)) return true;
     return false;
 }
 
@@ -598,7 +619,7 @@ __device__ __forceinline__ void face_nb(const StepArgs& a, int f, int l, int m, 
 // segment applied, or -1.  The condition loop stays rolled: one copy of the
 // interpreter per call site.
 __device__ __forceinline__ int face_chain(const StepArgs& a, const WorkArea& w, int f, int l, int m, int seg_end) {
-    const MpdBcProgram* bc = a.bc;
+    const BcImage* bc = img(a);
     const bool is_l = (f == MPD_L_LO || f == MPD_L_HI);
     const int i = is_l ? m : l;
     // The geometry symbols, read before the segment loop so that their loads
@@ -609,7 +630,7 @@ __device__ __forceinline__ int face_chain(const StepArgs& a, const WorkArea& w, 
     const double g_rnb = cell_r(a, ln, mn), g_rznb = cell_rz(a, ln, mn);
     int last = -1;
     for (int s = 0; s < seg_end; ++s) {
-        const MpdSeg& g = bc->seg[f][s];
+        const MpdSeg& g = bc->seg(f, s);
         if (!seg_covers(g, i)) continue;
         last = s;
         if (g.has_expr) {
@@ -673,9 +694,9 @@ __device__ __forceinline__ void mlo_plan(const StepArgs& a, int l, int s, bool w
     p.nl = l;
     p.nm = 2;
     if (l == 1) {
-        if (face_covers(a.bc, MPD_L_LO, 1)) {
+        if (face_covers(img(a), MPD_L_LO, 1)) {
             p.cf = MPD_L_LO;
-            p.ce = a.bc->nseg[MPD_L_LO];
+            p.ce = img(a)->nseg[MPD_L_LO];
             p.nl = 2;
             p.nm = 1;
         }
@@ -759,11 +780,11 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
     int ln, mn;
     face_nb(a, f, l, m, ln, mn);
     int pf = -1, pl = 0, pm = 0, ql = 0, qm = 0;  // earlier face at the neighbour, its cell and neighbour
-    if (f == MPD_M_HI && ln == 1 && face_covers(a.bc, MPD_L_LO, mn)) {
+    if (f == MPD_M_HI && ln == 1 && face_covers(img(a), MPD_L_LO, mn)) {
         pf = MPD_L_LO; pl = 1; pm = mn; ql = 2; qm = mn;
-    } else if (f == MPD_L_HI && mn == 1 && face_covers(a.bc, MPD_M_LO, ln)) {
+    } else if (f == MPD_L_HI && mn == 1 && face_covers(img(a), MPD_M_LO, ln)) {
         pf = MPD_M_LO; pl = ln; pm = 1; ql = ln; qm = 2;
-    } else if (f == MPD_L_HI && mn == a.Mn && face_covers(a.bc, MPD_M_HI, ln)) {
+    } else if (f == MPD_L_HI && mn == a.Mn && face_covers(img(a), MPD_M_HI, ln)) {
         pf = MPD_M_HI; pl = ln; pm = a.Mn; ql = ln; qm = a.Mn - 1;
     }
     // Every load of the chain first: own state (only AxisLF conditions read
@@ -782,17 +803,17 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
         src_phys(sq, vq, cell_r(a, ql, qm), a.gm1, nb2);
         w.put(WA_CUR, nb);
         w.put(WA_NBR, nb2);
-        face_chain(a, w, pf, pl, pm, a.bc->nseg[pf]);
+        face_chain(a, w, pf, pl, pm, img(a)->nseg[pf]);
         w.get(WA_CUR, nb);
     }
     w.put(WA_CUR, st);
     w.put(WA_NBR, nb);
-    const int last = face_chain(a, w, f, l, m, a.bc->nseg[f]);
+    const int last = face_chain(a, w, f, l, m, img(a)->nseg[f]);
     w.get(WA_CUR, st);
     const double r = cell_r(a, l, m);
     rebuild_u(st, r, u);
-    if (f == MPD_M_LO && last >= 0 && a.bc->seg[f][last].has_axis) {
-        const MpdSeg& g = a.bc->seg[f][last];
+    if (f == MPD_M_LO && last >= 0 && img(a)->seg(f, last).has_axis) {
+        const MpdSeg& g = img(a)->seg(f, last);
         MloPlan plo, phi;
         mlo_plan(a, l - 1, last, seg_covers(g, l - 1), plo);
         mlo_plan(a, l + 1, last, false, phi);
@@ -834,11 +855,11 @@ __device__ void finalize_step(const StepArgs& a) {
     st->ticket = 0;
 }
 
-// Copies the parts of the BC program in use into `prog` (LDS, same layout):
-// the header, the used segments of each face and the expression code.  The
-// header is read first; then each thread issues all its loads before its LDS
-// stores, so the copy costs two global round trips (a plain strided copy
-// loop waits for each load in turn: 10 us measured at 64 threads).
+// Builds the LDS image of the BC program (BcImage) in `prog`: the header
+// from the counts (read first), then each thread issues all its loads of the
+// segments and code in use before its LDS stores, so the copy costs two
+// global round trips (a plain strided copy loop waits for each load in turn:
+// 10 us measured at 64 threads).
 __device__ __forceinline__ void stage_program(const MpdBcProgram* bc, uint64_t* prog) {
     constexpr int kHead = (int)(offsetof(MpdBcProgram, seg) / 8);
     constexpr int kSegW = (int)(sizeof(MpdSeg) / 8);
@@ -846,34 +867,49 @@ __device__ __forceinline__ void stage_program(const MpdBcProgram* bc, uint64_t* 
     constexpr int kCode = (int)(offsetof(MpdBcProgram, code) / 8);
     static_assert(offsetof(MpdBcProgram, seg) % 8 == 0 && sizeof(MpdSeg) % 8 == 0 && sizeof(MpdInstr) == 16, "");
     const uint64_t* src = reinterpret_cast<const uint64_t*>(bc);
-    const int w0 = kSegW * bc->nseg[0];
-    const int w1 = w0 + kSegW * bc->nseg[1];
-    const int w2 = w1 + kSegW * bc->nseg[2];
-    const int w3 = w2 + kSegW * bc->nseg[3];
-    const int total = kHead + w3 + 2 * bc->ncode;
+    const int n0 = bc->nseg[0], n1 = bc->nseg[1], n2 = bc->nseg[2], n3 = bc->nseg[3];
+    const int ncode = bc->ncode;
+    if (threadIdx.x == 0) {
+        BcImage* h = reinterpret_cast<BcImage*>(prog);
+        h->nseg[0] = n0;
+        h->nseg[1] = n1;
+        h->nseg[2] = n2;
+        h->nseg[3] = n3;
+        h->ncode = ncode;
+        h->first[0] = 0;
+        h->first[1] = n0;
+        h->first[2] = n0 + n1;
+        h->first[3] = n0 + n1 + n2;
+        h->nsegs = n0 + n1 + n2 + n3;
+    }
+    const int w0 = kSegW * n0;
+    const int w1 = w0 + kSegW * n1;
+    const int w2 = w1 + kSegW * n2;
+    const int w3 = w2 + kSegW * n3;
+    const int total = w3 + 2 * ncode;
+    uint64_t* dst = prog + kBcImageHeadWords;
     constexpr int kPer = 8;
     for (int base = 0; base < total; base += kPer * BC_THREADS) {
         uint64_t v[kPer];
         int d[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            int i = base + j * BC_THREADS + (int)threadIdx.x;
+            const int i = base + j * BC_THREADS + (int)threadIdx.x;
             d[j] = -1;
             if (i < total) {
                 int o;
-                if (i < kHead) o = i;
-                else if ((i -= kHead) < w0) o = kHead + i;
+                if (i < w0) o = kHead + i;
                 else if (i < w1) o = kHead + kFaceW + (i - w0);
                 else if (i < w2) o = kHead + 2 * kFaceW + (i - w1);
                 else if (i < w3) o = kHead + 3 * kFaceW + (i - w2);
                 else o = kCode + (i - w3);
-                d[j] = o;
+                d[j] = i;
                 v[j] = src[o];
             }
         }
 #pragma unroll
         for (int j = 0; j < kPer; ++j)
-            if (d[j] >= 0) prog[d[j]] = v[j];
+            if (d[j] >= 0) dst[d[j]] = v[j];
     }
     __syncthreads();
 }
@@ -888,7 +924,7 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt_ex
     const double dt = step_dt(a_in, dt_explicit, loop);
     stage_program(a_in.bc, prog);
     StepArgs a = a_in;
-    a.bc = reinterpret_cast<const MpdBcProgram*>(prog);
+    a.bc = reinterpret_cast<const MpdBcProgram*>(prog);  // the image, read through img(a)
     const WorkArea w{area, (int)threadIdx.x};
     const int L = a.L, Mn = a.Mn;
     const int nring = 2 * Mn + 2 * (L - 2);
@@ -903,7 +939,7 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt_ex
         // Last face covering the cell (M_LO skips l = 1 and l = L by its corner
         // policy, so it never owns a row-1 or row-L cell).
         int f = -1;
-        const MpdBcProgram* bc = a.bc;
+        const BcImage* bc = img(a);
         if (l == 1) {
             if (m == Mn && face_covers(bc, MPD_M_HI, 1)) f = MPD_M_HI;
             else if (face_covers(bc, MPD_L_LO, m)) f = MPD_L_LO;
@@ -966,10 +1002,11 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt_ex
 __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double dt_explicit, int loop, int speed) {
     if (loop && loop_idle(a_in)) return;
     // The BC program (segments and the expression code in use) is read by
-    // every thread many times over in dependent chains: copy it to LDS once.
-    __shared__ uint64_t prog[sizeof(MpdBcProgram) / 8];
-    __shared__ double area[WA_N * BC_THREADS];
-    bc_ring_block(a_in, dt_explicit, loop, speed, (int)blockIdx.x, (int)gridDim.x, prog, area);
+    // every thread many times over in dependent chains: its image goes to
+    // LDS once, the per-lane work area behind it.
+    extern __shared__ uint64_t bc_lds[];
+    bc_ring_block(a_in, dt_explicit, loop, speed, (int)blockIdx.x, (int)gridDim.x, bc_lds,
+                  reinterpret_cast<double*>(bc_lds + a_in.bc_words));
 }
 
 // The boundary kernel over many instances: work entry b = (instance, block
@@ -978,13 +1015,13 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
 __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring_batch(const StepArgs* __restrict__ args,
                                                                const int* __restrict__ icur,
                                                                const BatchBlock* __restrict__ work, int flip,
-                                                               int speed) {
+                                                               int speed, int img_words) {
     const BatchBlock w = work[blockIdx.x];
     const StepArgs a = args[2 * w.inst + (icur[w.inst] ^ flip)];
     if (loop_idle(a)) return;
-    __shared__ uint64_t prog[sizeof(MpdBcProgram) / 8];
-    __shared__ double area[WA_N * BC_THREADS];
-    bc_ring_block(a, 0.0, 1, speed, w.first, w.count, prog, area);
+    // img_words: the largest image of the launch's instances.
+    extern __shared__ uint64_t bc_lds[];
+    bc_ring_block(a, 0.0, 1, speed, w.first, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words));
 }
 
 // Exact max wave speed over the whole new state, run when the candidate max
@@ -1263,6 +1300,11 @@ __global__ void lf_fill(double* p, int64_t n, double v) {
 // ---------------------------------------------------------------- launchers
 
 namespace {
+// Dynamic LDS of a boundary workgroup: the program image and the work area.
+size_t bc_lds_bytes(int img_words, int area_slots) {
+    return ((size_t)img_words + (size_t)area_slots * BC_THREADS) * 8;
+}
+
 template <class Div>
 hipError_t launch_simple(const StepArgs& a, double dt, int loop, int speed, hipStream_t s) {
     const dim3 blk(64, 4);
@@ -1317,8 +1359,8 @@ hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt, int
 
 hipError_t launch_bc(const StepArgs& a, double dt, int loop, int speed, hipStream_t s) {
     const int nring = 2 * a.Mn + 2 * (a.L - 2);
-    hipLaunchKernelGGL(lf_bc_ring, dim3((unsigned)((nring + BC_THREADS - 1) / BC_THREADS)), dim3(BC_THREADS), 0, s, a,
-                       dt, loop, speed);
+    hipLaunchKernelGGL(lf_bc_ring, dim3((unsigned)((nring + BC_THREADS - 1) / BC_THREADS)), dim3(BC_THREADS),
+                       bc_lds_bytes(a.bc_words, a.bc_area), s, a, dt, loop, speed);
     return hipGetLastError();
 }
 
@@ -1337,9 +1379,11 @@ hipError_t launch_stencil_batch(const StepArgs* args, const int* icur, const Bat
 }
 
 hipError_t launch_bc_batch(const StepArgs* args, const int* icur, const BatchBlock* work, int nwork, int flip,
-                           hipStream_t s) {
+                           int img_words, int area_slots, hipStream_t s) {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(lf_bc_ring_batch, dim3((unsigned)nwork), dim3(BC_THREADS), 0, s, args, icur, work, flip, 1);
+    const size_t lds = bc_lds_bytes(img_words, area_slots);
+    hipLaunchKernelGGL(lf_bc_ring_batch, dim3((unsigned)nwork), dim3(BC_THREADS), lds, s, args, icur, work, flip, 1,
+                       img_words);
     return hipGetLastError();
 }
 

@@ -50,7 +50,7 @@ int residual_blocks(const StepArgs& a);
 hipError_t launch_stencil_batch(const StepArgs* args, const int* icur, const BatchBlock* work, int nwork, int grid,
                                 int flip, bool fast_div, bool buf, double dt_explicit, int loop, hipStream_t s);
 hipError_t launch_bc_batch(const StepArgs* args, const int* icur, const BatchBlock* work, int nwork, int flip,
-                           hipStream_t s);
+                           int img_words, int area_slots, hipStream_t s);
 hipError_t launch_residual_batch(const StepArgs* args, const int* icur, const BatchAux* aux, const BatchBlock* work,
                                  int nwork, int flip, hipStream_t s);
 // Tiles of one instance for tile height H (strips of central columns x row chunks).

@@ -62,7 +62,7 @@ struct MpdSeg {
 struct MpdBcProgram {
     int32_t nseg[4];
     int32_t ncode;
-    int32_t pad;
+    int32_t max_stack;  // deepest evaluation stack of the program's expressions
     MpdSeg seg[4][MPD_MAX_SEGS_PER_FACE];
     MpdInstr code[MPD_MAX_CODE];
 };

@@ -181,20 +181,30 @@ __global__ void reduce_splits(const float* part, float* dw, int64_t n, int S) {
     }
 }
 
-// Bias gradient: db[c] = sum over pixels of dz[p][c] (one block per channel,
-// fixed-order tree).
-__global__ void __launch_bounds__(256) bias_grad(const float* dz, float* db, int P, int C) {
-    const int c = (int)blockIdx.x;
+// Bias gradient, pass 1: block (cb, s) sums channels 64 cb .. 64 cb + 63
+// over pixel slice s; the 4 waves take interleaved pixels, lane = channel,
+// so each wave reads 64 consecutive floats per pixel.  part[s][c].
+__global__ void __launch_bounds__(256) bias_grad_part(const float* dz, float* part, int P, int C, int rows) {
+    const int c = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    const int w = (int)(threadIdx.x >> 6);
+    const int p0 = (int)blockIdx.y * rows, p1 = min(P, p0 + rows);
     float s = 0.f;
-    for (int p = (int)threadIdx.x; p < P; p += 256) s += dz[(int64_t)p * C + c];
-    __shared__ float sh[256];
-    sh[threadIdx.x] = s;
+    if (c < C)
+        for (int p = p0 + w; p < p1; p += 4) s += dz[(int64_t)p * C + c];
+    __shared__ float sh[4][64];
+    sh[w][threadIdx.x & 63] = s;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) db[c] = sh[0];
+    if (w == 0 && c < C) part[(int64_t)blockIdx.y * C + c] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) +
+                                                             (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+}
+
+// Pass 2: db[c] = sum over slices in order (deterministic).
+__global__ void bias_grad_sum(const float* part, float* db, int C, int S) {
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= C) return;
+    float s = 0.f;
+    for (int k = 0; k < S; ++k) s += part[(int64_t)k * C + c];
+    db[c] = s;
 }
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
@@ -383,6 +393,12 @@ bool conv_dims_ok(int B, int H, int W, int Cin, int Cout) {
            (int64_t)9 * Cin < (1 << 20) && (int64_t)9 * Cout < (1 << 20);
 }
 
+// Pixel slices of the bias gradient: about 512 pixels each, at most 256.
+int bias_slices(int P, int C) {
+    (void)C;
+    return std::max(1, std::min(256, P / 512));
+}
+
 int wgrad_splits(int P) { return std::max(1, std::min(64, P / 2048)); }
 int wgrad_chunk(int P, int S) { return ((P + S - 1) / S + BK - 1) / BK * BK; }
 
@@ -477,10 +493,20 @@ int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, 
     return finish(hipSuccess);
 }
 
-int mpd_nn_bias_grad(const float* dz, float* db, int P, int C, void* stream) {
-    if (!dz || !db || P <= 0 || C <= 0) return bad("bias_grad: bad argument");
-    hipLaunchKernelGGL(bias_grad, dim3((unsigned)C), dim3(256), 0, (hipStream_t)stream, dz, db, P, C);
+int mpd_nn_bias_grad(const float* dz, float* db, float* work, int P, int C, void* stream) {
+    if (!dz || !db || !work || P <= 0 || C <= 0) return bad("bias_grad: bad argument");
+    const int S = bias_slices(P, C);
+    const int rows = (P + S - 1) / S;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(bias_grad_part, dim3((unsigned)((C + 63) / 64), (unsigned)S), dim3(256), 0, s, dz, work, P, C,
+                       rows);
+    hipLaunchKernelGGL(bias_grad_sum, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, s, (const float*)work, db, C, S);
     return finish(hipSuccess);
+}
+
+long long mpd_nn_bias_work_floats(int P, int C) {
+    if (P <= 0 || C <= 0) return -1;
+    return (long long)bias_slices(P, C) * C;
 }
 
 int mpd_nn_silu_fwd(const float* z, uint16_t* a, long long n, void* stream) {

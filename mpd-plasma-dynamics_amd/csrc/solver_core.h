@@ -130,6 +130,7 @@ inline void CompileBc(const mpd::SimConfig& cfg, MpdBcProgram& bc) {
                             throw std::runtime_error("boundary_conditions: expressions too long in total");
                         c.code_off = ncode;
                         c.code_len = (int)e.code.size();
+                        bc.max_stack = std::max(bc.max_stack, e.max_stack);
                         for (const MpdInstr& in : e.code) bc.code[ncode++] = in;
                         break;
                     }
@@ -221,6 +222,8 @@ struct mpd_solver {
     // state in the batch's array of loop states (neither owned here).
     bool own_stream = true, own_st = true;
     bool in_batch = false;  // owned by an mpd_batch (mpd_destroy leaves it alone)
+    int bc_words = 0;       // LDS image size of the BC program (8-byte words)
+    int bc_area = 0;        // per-lane work-area slots of the boundary kernel
 
     ~mpd_solver() {
         for (auto& g : graph)
@@ -266,6 +269,8 @@ struct mpd_solver {
         a.T = cfg.T;
         a.adaptive = cfg.adaptive_dt ? 1 : 0;
         a.step1 = step1 ? 1 : 0;
+        a.bc_words = bc_words;
+        a.bc_area = bc_area;
         a.st = st;
         a.bc = bc;
         return a;
@@ -394,6 +399,8 @@ struct mpd_solver {
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipMalloc((void**)&bc, sizeof(MpdBcProgram)));
         HIPCHK(hipMemcpy(bc, &hbc, sizeof(MpdBcProgram), hipMemcpyHostToDevice));
+        bc_words = mpd::bc_image_words(hbc);
+        bc_area = mpd::bc_area_slots(hbc);
         if (ext_st) {
             st = ext_st;
             own_st = false;

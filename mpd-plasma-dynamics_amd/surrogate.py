@@ -54,7 +54,8 @@ def nn_lib():
         "mpd_nn_conv3x3_bwd_data": [_vp, _vp, _vp, i, i, i, i, i, _vp],
         "mpd_nn_conv3x3_bwd_weight": [_vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
         "mpd_nn_wgrad_work_floats": [i, i, i, i, i],
-        "mpd_nn_bias_grad": [_vp, _vp, i, i, _vp],
+        "mpd_nn_bias_grad": [_vp, _vp, _vp, i, i, _vp],
+        "mpd_nn_bias_work_floats": [i, i],
         "mpd_nn_silu_fwd": [_vp, _vp, ll, _vp],
         "mpd_nn_silu_bwd": [_vp, _vp, _vp, _vp, ll, _vp],
         "mpd_nn_to_bf16": [_vp, _vp, ll, _vp],
@@ -79,7 +80,7 @@ def nn_lib():
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
-        fn.restype = C.c_longlong if name == "mpd_nn_wgrad_work_floats" else C.c_int
+        fn.restype = C.c_longlong if name.endswith("_work_floats") else C.c_int
     L.mpd_nn_last_error.restype = C.c_char_p
     _nn = L
     return L
@@ -87,7 +88,7 @@ def nn_lib():
 
 # Every symbol include/mpd_nn.h declares.
 NN_EXPORTS = ["mpd_nn_last_error", "mpd_nn_conv3x3_fwd", "mpd_nn_conv3x3_bwd_data", "mpd_nn_wgrad_work_floats",
-              "mpd_nn_conv3x3_bwd_weight", "mpd_nn_bias_grad", "mpd_nn_silu_fwd", "mpd_nn_silu_bwd",
+              "mpd_nn_conv3x3_bwd_weight", "mpd_nn_bias_work_floats", "mpd_nn_bias_grad", "mpd_nn_silu_fwd", "mpd_nn_silu_bwd",
               "mpd_nn_to_bf16", "mpd_nn_avgpool2_fwd", "mpd_nn_avgpool2_bwd", "mpd_nn_upsample2_fwd",
               "mpd_nn_upsample2_bwd", "mpd_nn_concat_fwd", "mpd_nn_concat_bwd", "mpd_nn_add", "mpd_nn_mse_blocks",
               "mpd_nn_mse", "mpd_nn_adam", "mpd_nn_gather_rows", "mpd_nn_malloc", "mpd_nn_free",
@@ -275,6 +276,8 @@ class UNet:
         L = nn_lib()
         work = max(L.mpd_nn_wgrad_work_floats(B, d["h"], d["w"], d["ci"], d["co"]) for d in self.lay)
         self.work = DeviceArray(int(work))
+        bwork = max(L.mpd_nn_bias_work_floats(B * d["h"] * d["w"], d["co"]) for d in self.lay)
+        self.bwork = DeviceArray(int(bwork))
         big = max(B * d["h"] * d["w"] * max(d["ci"], d["co"]) for d in self.lay)
         # Gradient scratch: two ping-pong FP32 buffers, bf16 dz, FP32 dz, and
         # per-skip gradient buffers.
@@ -346,7 +349,7 @@ class UNet:
         else:
             _ck(L.mpd_nn_to_bf16(_p(da), _p(self.dzb), n, s))
             dzf = da
-        _ck(L.mpd_nn_bias_grad(_p(dzf), _p(self.gb[k]), B * d["h"] * d["w"], d["co"], s))
+        _ck(L.mpd_nn_bias_grad(_p(dzf), _p(self.gb[k]), _p(self.bwork), B * d["h"] * d["w"], d["co"], s))
         _ck(L.mpd_nn_conv3x3_bwd_weight(_p(d["a_in"]), _p(self.dzb), _p(self.gw[k]), _p(self.work), B, d["h"],
                                         d["w"], d["ci"], d["co"], s))
         if dx_out is not None:

@@ -153,6 +153,15 @@ def test_elementwise_kernels_match_oracle(nn):
     np.testing.assert_allclose(_host(Wd), rw, rtol=1e-5, atol=1e-6)
     assert np.array_equal(_host_bf16(WB), S.bf16_round(_host(Wd)))
 
+    for P, C in ((5, 3), (3000, 70), (40000, 8)):
+        dzb = rng.normal(size=(P, C)).astype(np.float32)
+        DB = _empty((C,))
+        work = _empty((int(L.mpd_nn_bias_work_floats(P, C)),))
+        nn._ck(L.mpd_nn_bias_grad(nn._p(_dev_f32(dzb)), nn._p(DB), nn._p(work), P, C, s))
+        ref = dzb.astype(np.float64).sum(axis=0)
+        mag = np.abs(dzb.astype(np.float64)).sum(axis=0)
+        assert np.all(np.abs(_host(DB) - ref) <= P * 2.0 ** -23 * mag)
+
     src = rng.normal(size=(6, 4)).astype(np.float32)
     idx = np.array([5, 0, 3], np.int32)
     dst = _empty((3, 4))
