@@ -1,5 +1,10 @@
 """Benchmark of the FP64 Lax-Friedrichs step on one MI355X.
 
+Also on the line: `batched_sweep` (64 uneven instances at the sweep sizes, batched
+against one at a time) and `steady_state` (a reduced live run of the learned warm
+start: wall time to residual 1e-8 of surrogate-warm-started batched solves against
+cold-started plain solves; --no-warmstart skips it).
+
 Workload: the memory-bound scaling case, the converging-diverging channel
 r_in = 0.2 + 0.05 sin(pi z), r_out = 0.8 - 0.1 z^2 at 6400 x 2560 (L_max x
 M_max; 16,390,400 interior cells), IC rho = 1 (+1e-3 seeded noise), u_z = 0.1,
@@ -48,6 +53,73 @@ def cpu_baseline(L, M, steps):
                       f"({dt:.1f} s), oracle/lf_oracle.c with OpenMP"}
 
 
+def sweep_measure(steps=1000, warm=200, n=64, seed=12345):
+    """Batched sweep at the uneven sizes nz in {200, 400, 800, 1600}, nr = 0.4 nz
+    (n seeded inflow cases): per-step time of the batch, against the sum of
+    the per-step times of single solves of each size (one case at a time)."""
+    import numpy as np
+    import mpd_amd
+    import sweep
+    rng = np.random.default_rng(seed)
+    sizes, cfgs = [], []
+    for _ in range(n):
+        nz = int(rng.choice([200, 400, 800, 1600]))
+        q = [rng.uniform(0.5, 2.0), rng.uniform(0.05, 0.3), rng.uniform(0.05, 0.5), 0.05, 0.1]
+        sizes.append(nz)
+        cfgs.append(sweep.case_config(q, nz, int(nz * 0.4), threshold=0.0, max_steps=0))
+    cells = sum(nz * (int(nz * 0.4) + 1) for nz in sizes)
+    with mpd_amd.Batch(cfgs) as b:
+        b.run(warm)
+        b.autotune(5)
+        t0 = time.perf_counter()
+        b.run(steps)
+        batch_s = (time.perf_counter() - t0) / steps
+    single = {}
+    for nz in sorted(set(sizes)):
+        with mpd_amd.Solver(cfgs[sizes.index(nz)]) as sol:
+            sol.run(warm)
+            t0 = time.perf_counter()
+            sol.run(steps)
+            single[nz] = (time.perf_counter() - t0) / steps
+    seq_s = sum(single[nz] for nz in sizes)
+    return {"workload": f"{n} instances, nz in {{200,400,800,1600}} (counts "
+                        f"{[sizes.count(z) for z in (200, 400, 800, 1600)]}), nr = 0.4 nz, {cells} cells",
+            "batched_ms_per_step": round(batch_s * 1e3, 4), "batched_gcups": round(cells / batch_s / 1e9, 3),
+            "one_at_a_time_ms_per_step": round(seq_s * 1e3, 4), "one_at_a_time_gcups": round(cells / seq_s / 1e9, 3),
+            "speedup": round(seq_s / batch_s, 2), "steps_timed": steps}
+
+
+def warmstart_measure(train, test, epochs, L=96, M=39):
+    """Reduced live run of the learned warm start: train on `train` solver
+    steady states, then time plain cold solves (8 timed, scaled), cold and
+    warm batched solves of `test` held-out cases to residual 1e-8."""
+    import warmstart as WS
+    import surrogate as SG
+    import sweep
+    import numpy as np
+    t0 = time.perf_counter()
+    params, fields, steps, conv = WS.generate(train, L, M, seed=2024, max_steps=300000, batch=512)
+    gen_s = time.perf_counter() - t0
+    params, fields = params[conv], fields[conv]
+    radii = WS.grid_radii(params, L, M)
+    base = SG.baseline_fields(params, radii, L, M, WS.GAMMA)
+    norm = WS.Normalizer(fields - base)
+    net = SG.UNet(L, M + 1, seed=2024)
+    t1 = time.perf_counter()
+    losses = WS.train(net, SG.features(params, radii, L, M), norm.encode(fields - base).astype(np.float32), epochs,
+                      32, 2e-3, log_every=max(1, epochs // 4))
+    train_s = time.perf_counter() - t1
+    r, _, _ = WS.evaluate(net, norm, sweep.sample_params(test, 2025), L, M, plain_sample=8)
+    keep = ["plain_cold_total_s", "cold_batch_s", "warm_total_s", "speedup_vs_plain_cold", "speedup_vs_cold_batch",
+            "steps_ratio_cold_over_warm", "fallbacks", "all_converged_warm", "warm_finish_p50_s",
+            "warm_finish_p99_s", "max_field_err_warm_vs_cold"]
+    out = {k: r[k] for k in keep}
+    out.update({"workload": f"{test} held-out sweep cases at {L}x{M}, residual 1e-8; network trained on "
+                            f"{int(conv.sum())} steady states, {epochs} epochs",
+                "generate_s": round(gen_s, 2), "train_s": round(train_s, 2), "final_loss": losses[-1]})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,8 +127,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--L", type=int, default=6400)
     ap.add_argument("--M", type=int, default=2560)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--warmstart-train", type=int, default=500)
+    ap.add_argument("--warmstart-test", type=int, default=64)
+    ap.add_argument("--warmstart-epochs", type=int, default=60)
+    ap.add_argument("--no-warmstart", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
@@ -101,6 +178,12 @@ def main():
         "step_breakdown_ms": {"stencil": round(tm["stencil_ms"], 5), "boundary": round(tm["bc_ms"], 5)},
         "wall_s": round(wall, 3),
     }
+    if not args.no_sweep:
+        print("bench: batched sweep", file=sys.stderr, flush=True)
+        line["batched_sweep"] = sweep_measure()
+    if not args.no_warmstart:
+        print("bench: learned warm start (reduced live run)", file=sys.stderr, flush=True)
+        line["steady_state"] = warmstart_measure(args.warmstart_train, args.warmstart_test, args.warmstart_epochs)
     if not args.no_cpu:
         print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(L, M, args.cpu_steps)

@@ -27,7 +27,7 @@ GAMMA = 1.6666666666666667
 
 
 def progress(msg):
-    print(f"[warmstart {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+    print(f"[warmstart {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def grid_radii(params, L, M):
