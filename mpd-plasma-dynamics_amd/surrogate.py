@@ -472,3 +472,37 @@ def u0_from_fields(f, r, rows, cols):
         a[1:L_max + 1, 1:Mn + 1] = v
         out.append(a)
     return out
+
+
+def main():
+    """Times training steps (forward, MSE, backward, Adam) on random data."""
+    import argparse
+    import time
+    ap = argparse.ArgumentParser(description="time training steps of the warm-start network")
+    ap.add_argument("--H", type=int, default=96)
+    ap.add_argument("--W", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=40)
+    args = ap.parse_args()
+    H, W, B = args.H, args.W, args.batch
+    net = UNet(H, W, seed=0)
+    rng = np.random.default_rng(0)
+    x = DeviceArray.from_numpy(rng.normal(size=(B, H, W, N_FEATURES)), "bf16")
+    t = DeviceArray.from_numpy(rng.normal(size=(B, H, W, len(OUT_FIELDS))))
+    dy = DeviceArray((B, H, W, len(OUT_FIELDS)))
+    warm = min(10, args.steps - 1)
+    for it in range(args.steps):
+        if it == warm:
+            _ck(nn_lib().mpd_nn_synchronize())
+            t0 = time.perf_counter()
+        y = net.forward(x, B)
+        net.mse(y, t, dy)
+        net.backward(dy, B)
+        net.adam_step(1e-3)
+    _ck(nn_lib().mpd_nn_synchronize())
+    print(f"training step, batch {B} at {H}x{W}: {(time.perf_counter() - t0) / (args.steps - warm) * 1e3:.2f} ms",
+          flush=True)
+
+
+if __name__ == "__main__":
+    main()

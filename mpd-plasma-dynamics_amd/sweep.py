@@ -119,3 +119,42 @@ def run_batch(mpd, configs, warm_u0=None, max_steps=0, autotune=True):
     t0 = time.perf_counter()
     res = b.run(max_steps)
     return res, time.perf_counter() - t0, b
+
+
+def probe_configs(kind, seed=7):
+    """The profiling workloads: 'small' = 256 sweep cases at 96x39, 'uneven' =
+    64 cases with nz in {200, 400, 800, 1600}, nr = 0.4 nz (cold starts,
+    convergence checks every 100 steps at threshold 1e-8)."""
+    if kind == "small":
+        return [case_config(q, 96, 39) for q in sample_params(256, seed)]
+    rng = np.random.default_rng(12345)
+    cfgs = []
+    for _ in range(64):
+        nz = int(rng.choice([200, 400, 800, 1600]))
+        q = [rng.uniform(0.5, 2.0), rng.uniform(0.05, 0.3), rng.uniform(0.05, 0.5), 0.05, 0.1]
+        cfgs.append(case_config(q, nz, int(nz * 0.4)))
+    return cfgs
+
+
+def main():
+    import argparse
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import mpd_amd
+    ap = argparse.ArgumentParser(description="time a batched sweep workload")
+    ap.add_argument("--probe", choices=["small", "uneven"], required=True)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    args = ap.parse_args()
+    with mpd_amd.Batch(probe_configs(args.probe)) as b:
+        b.run(args.warmup)
+        print(b.autotune(5), flush=True)
+        t0 = time.perf_counter()
+        b.run(args.steps)
+        el = time.perf_counter() - t0
+    print(f"{args.probe}: {args.steps} steps {el:.3f} s = {el / args.steps * 1e6:.1f} us/step", flush=True)
+
+
+if __name__ == "__main__":
+    main()
