@@ -98,6 +98,18 @@ __device__ __forceinline__ uint16_t load_b(const ConvArgs& a, int n, int k) {
     }
 }
 
+// Staging order of the operand tiles: element e of a 64 x 32 tile is (row
+// e / 32, column e % 32) when the source is contiguous along k (consecutive
+// threads read consecutive k), (row e % 64, column e / 64) when it is
+// contiguous along the tile's rows (consecutive output channels / input
+// channels).  Forward: A and B contiguous in k.  Data gradient: A (dz) in k,
+// B (weights, n = input channel) in rows.  Weight gradient: A (dz, m =
+// output channel) and B (input, n = (tap, input channel)) in rows.
+template <int MODE>
+__device__ __forceinline__ constexpr bool a_rows_fast() { return MODE == MODE_WGRAD; }
+template <int MODE>
+__device__ __forceinline__ constexpr bool b_rows_fast() { return MODE != MODE_FWD; }
+
 template <int MODE>
 __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
     __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDP];
@@ -122,17 +134,18 @@ __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int e = t + 256 * j;
-            const int r = e >> 5, c = e & 31;
-            const int k = k0 + c;
-            va[j] = (k < k_end) ? load_a<MODE>(a, m0 + r, k) : (uint16_t)0;
-            vb[j] = (k < k_end) ? load_b<MODE>(a, n0 + r, k) : (uint16_t)0;
+            const int ra = a_rows_fast<MODE>() ? (e & 63) : (e >> 5), ca = a_rows_fast<MODE>() ? (e >> 6) : (e & 31);
+            const int rb = b_rows_fast<MODE>() ? (e & 63) : (e >> 5), cb = b_rows_fast<MODE>() ? (e >> 6) : (e & 31);
+            va[j] = (k0 + ca < k_end) ? load_a<MODE>(a, m0 + ra, k0 + ca) : (uint16_t)0;
+            vb[j] = (k0 + cb < k_end) ? load_b<MODE>(a, n0 + rb, k0 + cb) : (uint16_t)0;
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int e = t + 256 * j;
-            const int r = e >> 5, c = e & 31;
-            As[r * LDP + c] = va[j];
-            Bs[r * LDP + c] = vb[j];
+            const int ra = a_rows_fast<MODE>() ? (e & 63) : (e >> 5), ca = a_rows_fast<MODE>() ? (e >> 6) : (e & 31);
+            const int rb = b_rows_fast<MODE>() ? (e & 63) : (e >> 5), cb = b_rows_fast<MODE>() ? (e >> 6) : (e & 31);
+            As[ra * LDP + ca] = va[j];
+            Bs[rb * LDP + cb] = vb[j];
         }
         __syncthreads();
 #pragma unroll
@@ -198,13 +211,18 @@ __global__ void __launch_bounds__(256) bias_grad_part(const float* dz, float* pa
                                                              (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
 }
 
-// Pass 2: db[c] = sum over slices in order (deterministic).
-__global__ void bias_grad_sum(const float* part, float* db, int C, int S) {
-    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (c >= C) return;
+// Pass 2: db[c] = sum over slices (the 4 waves take interleaved slices,
+// then a fixed-order combine: deterministic).
+__global__ void __launch_bounds__(256) bias_grad_sum(const float* part, float* db, int C, int S) {
+    const int c = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    const int w = (int)(threadIdx.x >> 6);
     float s = 0.f;
-    for (int k = 0; k < S; ++k) s += part[(int64_t)k * C + c];
-    db[c] = s;
+    if (c < C)
+        for (int k = w; k < S; k += 4) s += part[(int64_t)k * C + c];
+    __shared__ float sh[4][64];
+    sh[w][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (w == 0 && c < C) db[c] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
 }
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
@@ -399,7 +417,13 @@ int bias_slices(int P, int C) {
     return std::max(1, std::min(256, P / 512));
 }
 
-int wgrad_splits(int P) { return std::max(1, std::min(64, P / 2048)); }
+// Pixel splits of the weight gradient: enough workgroups to fill the GPU
+// (about 2048 over the output tiles), each split at least 256 pixels.
+int wgrad_splits(int P, int M, int N) {
+    const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+    const int want = (2048 + tiles - 1) / tiles;
+    return std::max(1, std::min(want, P / 256));
+}
 int wgrad_chunk(int P, int S) { return ((P + S - 1) / S + BK - 1) / BK * BK; }
 
 }  // namespace
@@ -469,7 +493,7 @@ int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, in
 long long mpd_nn_wgrad_work_floats(int B, int H, int W, int Cin, int Cout) {
     if (!conv_dims_ok(B, H, W, Cin, Cout)) return -1;
     const int P = B * H * W;
-    return (long long)wgrad_splits(P) * Cout * 9 * Cin;
+    return (long long)wgrad_splits(P, Cout, 9 * Cin) * Cout * 9 * Cin;
 }
 
 int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, float* work, int B, int H, int W,
@@ -483,7 +507,7 @@ int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, 
     a.M = Cout;
     a.N = 9 * Cin;
     a.K = B * H * W;
-    const int S = wgrad_splits(a.K);
+    const int S = wgrad_splits(a.K, a.M, a.N);
     a.kchunk = wgrad_chunk(a.K, S);
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN), (unsigned)S);
     hipStream_t s = (hipStream_t)stream;
@@ -500,7 +524,7 @@ int mpd_nn_bias_grad(const float* dz, float* db, float* work, int P, int C, void
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(bias_grad_part, dim3((unsigned)((C + 63) / 64), (unsigned)S), dim3(256), 0, s, dz, work, P, C,
                        rows);
-    hipLaunchKernelGGL(bias_grad_sum, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, s, (const float*)work, db, C, S);
+    hipLaunchKernelGGL(bias_grad_sum, dim3((unsigned)((C + 63) / 64)), dim3(256), 0, s, (const float*)work, db, C, S);
     return finish(hipSuccess);
 }
 
