@@ -234,6 +234,7 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
             r = s.run()
             plain_t.append(time.perf_counter() - t0)
             plain_steps.append(r["steps"])
+        progress(f"plain cold solve {i + 1}/{m}: {r['steps']} steps, {plain_t[-1]:.2f}s")
     plain_total = float(np.sum(plain_t)) * (n / m)
     progress(f"plain cold solves: {m} timed, {np.mean(plain_t):.3f}s mean, {plain_total:.2f}s for {n}")
 
