@@ -18,11 +18,16 @@ initial condition (src/main.cpp:117-148).
 import ctypes as C
 import math
 import os
+import sys
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 NN_LIB_PATH = os.path.join(HERE, "libmpd_nn.so")
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+from sweep import PARAM_HI, PARAM_LO  # noqa: E402  (the sweep's parameter box)
 
 # Input features per pixel: the 5 parameters (scaled to [-1, 1]), z, the
 # radial grid fraction and the radius.
@@ -31,10 +36,6 @@ N_FEATURES = 8
 # names), in this order.
 OUT_FIELDS = ["rho", "v_z", "v_r", "v_phi", "e", "H_z", "H_r", "H_phi"]
 PHYS_INDEX = {"rho": 0, "v_z": 1, "v_r": 2, "v_phi": 3, "e": 4, "p": 5, "P": 6, "H_z": 7, "H_r": 8, "H_phi": 9}
-# Parameter ranges of the sweeps: rho_in, u_z_in, B_phi amplitude, inner and
-# outer wall amplitudes.
-PARAM_LO = np.array([0.5, 0.05, 0.05, 0.0, 0.0])
-PARAM_HI = np.array([2.0, 0.3, 0.5, 0.1, 0.1])
 
 _nn = None
 _vp = C.c_void_p
