@@ -271,10 +271,14 @@ struct mpd_batch {
         const int N = n();
         std::vector<int> step0(N);
         std::vector<char> active(N, 0);
+        // Every instance's loop state in one copy.
         for (int i = 0; i < N; ++i) {
             step0[i] = inst[i]->step;
-            active[i] = inst[i]->SeedLoop(max_steps) ? 0 : 1;
+            h_st.p[i] = inst[i]->LoopSeed(max_steps);
+            active[i] = h_st.p[i].done ? 0 : 1;
         }
+        HIPCHK(hipMemcpyAsync(st, h_st.p, sizeof(MpdLoopState) * N, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         // First steps from IC physical arrays: each instance on its own.
         bool any1 = false;
         for (int i = 0; i < N; ++i) {

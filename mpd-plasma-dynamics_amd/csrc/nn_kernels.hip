@@ -110,7 +110,87 @@ __device__ __forceinline__ constexpr bool a_rows_fast() { return MODE == MODE_WG
 template <int MODE>
 __device__ __forceinline__ constexpr bool b_rows_fast() { return MODE != MODE_FWD; }
 
+// Vector staging (every channel count a multiple of 8): each thread moves one
+// 8-element (16-byte) run of each operand tile per chunk.  The run lies along
+// k when the source is contiguous in k, else along the tile's rows; runs
+// never cross a tap (channel counts and run starts are multiples of 8), so a
+// run is wholly inside or wholly outside the image and the tile.
+union Run8 {
+    uint4 v;
+    uint16_t h[8];
+};
+
+__device__ __forceinline__ uint4 zero_run() { return uint4{0u, 0u, 0u, 0u}; }
+
+// The pixel of row m shifted by (dy, dx), or -1 outside the image.
+__device__ __forceinline__ int64_t shifted_pixel(const ConvArgs& a, int m, int dy, int dx) {
+    const int hw = a.H * a.W;
+    const int b = m / hw, r = m - b * hw;
+    const int y = r / a.W + dy, x = r % a.W + dx;
+    if (y < 0 || y >= a.H || x < 0 || x >= a.W) return -1;
+    return (int64_t)(b * a.H + y) * a.W + x;
+}
+
 template <int MODE>
+__device__ __forceinline__ uint4 vload_a(const ConvArgs& a, int m, int k) {
+    if (m >= a.M || k >= a.K) return zero_run();
+    if (MODE == MODE_FWD) {
+        const int tap = k / a.Cin, ci = k - tap * a.Cin;
+        const int64_t p = shifted_pixel(a, m, tap / 3 - 1, tap % 3 - 1);
+        return p < 0 ? zero_run() : *reinterpret_cast<const uint4*>(a.x + p * a.Cin + ci);
+    } else if (MODE == MODE_DGRAD) {
+        const int tap = k / a.Cout, co = k - tap * a.Cout;
+        const int64_t p = shifted_pixel(a, m, 1 - tap / 3, 1 - tap % 3);
+        return p < 0 ? zero_run() : *reinterpret_cast<const uint4*>(a.dz + p * a.Cout + co);
+    } else {
+        // rows m .. m + 7 (output channels) of pixel k
+        return *reinterpret_cast<const uint4*>(a.dz + (int64_t)k * a.Cout + m);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ uint4 vload_b(const ConvArgs& a, int n, int k) {
+    if (n >= a.N || k >= a.K) return zero_run();
+    if (MODE == MODE_FWD) {
+        return *reinterpret_cast<const uint4*>(a.w + (int64_t)n * a.K + k);
+    } else if (MODE == MODE_DGRAD) {
+        // rows n .. n + 7 (input channels) of weight column (tap, co)
+        const int tap = k / a.Cout, co = k - tap * a.Cout;
+        return *reinterpret_cast<const uint4*>(a.w + ((int64_t)co * 9 + tap) * a.Cin + n);
+    } else {
+        // rows n .. n + 7 = (tap, ci .. ci + 7) of pixel k
+        const int tap = n / a.Cin, ci = n - tap * a.Cin;
+        const int64_t p = shifted_pixel(a, k, tap / 3 - 1, tap % 3 - 1);
+        return p < 0 ? zero_run() : *reinterpret_cast<const uint4*>(a.x + p * a.Cin + ci);
+    }
+}
+
+// Thread t's run of a 64 x 32 tile: along k, row t / 4 and columns
+// 8 (t % 4) .. + 7; along rows, rows 8 (t % 8) .. + 7 and column t / 8.
+template <bool kRowsFast, class Load>
+__device__ __forceinline__ uint4 stage_load(int t, int r0, int k0, int k_end, Load ld) {
+    if (kRowsFast) {
+        const int k = k0 + (t >> 3);
+        return (k < k_end) ? ld(r0 + 8 * (t & 7), k) : zero_run();
+    }
+    const int k = k0 + 8 * (t & 3);
+    return (k < k_end) ? ld(r0 + (t >> 2), k) : zero_run();
+}
+
+template <bool kRowsFast>
+__device__ __forceinline__ void stage_store(int t, uint16_t* S, uint4 v) {
+    if (kRowsFast) {
+        Run8 u;
+        u.v = v;
+        const int c = t >> 3, r = 8 * (t & 7);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) S[(r + j) * LDP + c] = u.h[j];
+    } else {
+        *reinterpret_cast<uint4*>(&S[(t >> 2) * LDP + 8 * (t & 3)]) = v;
+    }
+}
+
+template <int MODE, bool VEC>
 __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
     __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDP];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDP];
@@ -128,8 +208,15 @@ __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int row = lane & 15, kq = (lane >> 4) * 4;
     for (int k0 = k_begin; k0 < k_end; k0 += BK) {
-        // Stage the 64 x 32 operand tiles: element e = t + 256 j of each,
-        // row e / 32, column e % 32 (consecutive threads, consecutive k).
+        if (VEC) {
+            const uint4 ra = stage_load<a_rows_fast<MODE>()>(t, m0, k0, k_end,
+                                                              [&](int m, int k) { return vload_a<MODE>(a, m, k); });
+            const uint4 rb = stage_load<b_rows_fast<MODE>()>(t, n0, k0, k_end,
+                                                              [&](int n, int k) { return vload_b<MODE>(a, n, k); });
+            stage_store<a_rows_fast<MODE>()>(t, As, ra);
+            stage_store<b_rows_fast<MODE>()>(t, Bs, rb);
+        } else {
+        // Stage the 64 x 32 operand tiles element by element (e = t + 256 j).
         uint16_t va[8], vb[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -146,6 +233,7 @@ __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
             const int rb = b_rows_fast<MODE>() ? (e & 63) : (e >> 5), cb = b_rows_fast<MODE>() ? (e >> 6) : (e & 31);
             As[ra * LDP + ca] = va[j];
             Bs[rb * LDP + cb] = vb[j];
+        }
         }
         __syncthreads();
 #pragma unroll
@@ -470,7 +558,8 @@ int mpd_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, 
     a.N = Cout;
     a.K = 9 * Cin;
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
-    hipLaunchKernelGGL(conv_gemm<MODE_FWD>, g, dim3(256), 0, (hipStream_t)stream, a);
+    if (Cin % 8 == 0) hipLaunchKernelGGL((conv_gemm<MODE_FWD, true>), g, dim3(256), 0, (hipStream_t)stream, a);
+    else hipLaunchKernelGGL((conv_gemm<MODE_FWD, false>), g, dim3(256), 0, (hipStream_t)stream, a);
     return finish(hipSuccess);
 }
 
@@ -486,7 +575,9 @@ int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, in
     a.N = Cin;
     a.K = 9 * Cout;
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
-    hipLaunchKernelGGL(conv_gemm<MODE_DGRAD>, g, dim3(256), 0, (hipStream_t)stream, a);
+    if (Cin % 8 == 0 && Cout % 8 == 0)
+        hipLaunchKernelGGL((conv_gemm<MODE_DGRAD, true>), g, dim3(256), 0, (hipStream_t)stream, a);
+    else hipLaunchKernelGGL((conv_gemm<MODE_DGRAD, false>), g, dim3(256), 0, (hipStream_t)stream, a);
     return finish(hipSuccess);
 }
 
@@ -511,7 +602,8 @@ int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, 
     a.kchunk = wgrad_chunk(a.K, S);
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN), (unsigned)S);
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(conv_gemm<MODE_WGRAD>, g, dim3(256), 0, s, a);
+    if (Cin % 8 == 0 && Cout % 8 == 0) hipLaunchKernelGGL((conv_gemm<MODE_WGRAD, true>), g, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((conv_gemm<MODE_WGRAD, false>), g, dim3(256), 0, s, a);
     const int64_t n = (int64_t)a.M * a.N;
     hipLaunchKernelGGL(reduce_splits, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)work, dw, n, S);
     return finish(hipSuccess);

@@ -570,8 +570,9 @@ struct mpd_solver {
         return h;
     }
 
-    // Seeds the device loop state from the host mirror; returns done.
-    bool SeedLoop(int max_steps) {
+    // The device loop state that continues the host mirror's loop for at
+    // most max_steps more steps (<= 0: no cap besides time.max_steps).
+    MpdLoopState LoopSeed(int max_steps) const {
         long long cap = LLONG_MAX;
         if (cfg.max_steps > 0) cap = cfg.max_steps;
         if (max_steps > 0) cap = std::min(cap, (long long)step + max_steps);
@@ -586,6 +587,12 @@ struct mpd_solver {
         h.converged = converged ? 1 : 0;
         h.done = (!(t < cfg.T) || converged || (h.max_steps > 0 && step >= h.max_steps)) ? 1 : 0;
         h.checked_step = checked_step;
+        return h;
+    }
+
+    // Seeds the device loop state from the host mirror; returns done.
+    bool SeedLoop(int max_steps) {
+        const MpdLoopState h = LoopSeed(max_steps);
         HIPCHK(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         return h.done != 0;
