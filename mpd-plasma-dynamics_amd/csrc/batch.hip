@@ -1,7 +1,8 @@
 // batch.hip -- batched parameter sweeps: many independent instances of the
 // LF time loop (each its own config.yaml: grid size, geometry, inflow, IC)
-// advanced together, one stencil launch, one boundary launch and one
-// convergence launch per step for all of them, K steps per HIP graph.
+// advanced together, one stencil launch, one boundary launch, one step-end
+// launch and one convergence launch per step for all of them, K steps per
+// HIP graph.
 //
 // Every instance keeps the semantics of a solver of its own (the reference's
 // time loop, src/main.cpp:191-247, per instance: its own dt policy, T,
@@ -222,6 +223,7 @@ struct mpd_batch {
     void EnqueueStep(int flip) {
         HIPCHK(mpd::launch_stencil_batch(d_args, d_icur, d_work, n_st, grid, flip, fast_div, buf, 0.0, 1, stream));
         HIPCHK(mpd::launch_bc_batch(d_args, d_icur, d_work + n_st, n_bc, flip, bc_words, bc_area, stream));
+        HIPCHK(mpd::launch_finish_batch(d_args, d_icur, n(), flip, stream));
         if (n_res > 0) HIPCHK(mpd::launch_residual_batch(d_args, d_icur, d_aux, d_work + n_st + n_bc, n_res, flip,
                                                          stream));
     }

@@ -918,9 +918,11 @@ __device__ __forceinline__ void stage_program(const MpdBcProgram* bc, uint64_t* 
 // Mn (l = 2..L-1).  Writes each cell's final conserved values, adds its wave
 // speed to the running max, and the last block finishes the step.
 // Block `bid` of the `nblk` blocks covering one instance's ring; prog and
-// area are the block's LDS.
+// area are the block's LDS.  finish: the last block to complete finishes the
+// loop step (ticket and device-scope fences); the batched path finishes its
+// instances' steps in a kernel of its own instead (lf_finish_batch).
 __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt_explicit, int loop, int speed, int bid,
-                                              int nblk, uint64_t* prog, double* area) {
+                                              int nblk, uint64_t* prog, double* area, bool finish) {
     const double dt = step_dt(a_in, dt_explicit, loop);
     stage_program(a_in.bc, prog);
     StepArgs a = a_in;
@@ -973,7 +975,7 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt_ex
         key = wave_max_u64(key);
         if ((threadIdx.x & 63) == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
     }
-    if (!loop) return;
+    if (!loop || !finish) return;
     // Last block finalises the loop state.
     __shared__ bool last;
     __syncthreads();
@@ -1006,7 +1008,7 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
     // LDS once, the per-lane work area behind it.
     extern __shared__ uint64_t bc_lds[];
     bc_ring_block(a_in, dt_explicit, loop, speed, (int)blockIdx.x, (int)gridDim.x, bc_lds,
-                  reinterpret_cast<double*>(bc_lds + a_in.bc_words));
+                  reinterpret_cast<double*>(bc_lds + a_in.bc_words), true);
 }
 
 // The boundary kernel over many instances: work entry b = (instance, block
@@ -1021,7 +1023,29 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring_batch(const StepArgs* _
     if (loop_idle(a)) return;
     // img_words: the largest image of the launch's instances.
     extern __shared__ uint64_t bc_lds[];
-    bc_ring_block(a, 0.0, 1, speed, w.first, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words));
+    bc_ring_block(a, 0.0, 1, speed, w.first, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words), false);
+}
+
+// The step's end for every instance of a batch that took it (after the
+// boundary kernel, so every wave-speed contribution of the step is in): the
+// exact-rescan test and the dt bookkeeping of lf_bc_ring's last block, one
+// thread per instance.  The kernel boundary orders it after all of the step's
+// atomics, so no per-workgroup ticket or device-scope fence is needed.
+__global__ void __launch_bounds__(64) lf_finish_batch(const StepArgs* __restrict__ args, const int* __restrict__ icur,
+                                                      int n, int flip) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const StepArgs a = args[2 * i + (icur[i] ^ flip)];
+    if (loop_idle(a)) return;
+    const double T = speed_T(a, 1);
+    const uint64_t bits = __hip_atomic_load(&a.st->speed_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double mc = __longlong_as_double((long long)bits);
+    if (T > 0.0 && !(mc >= T)) {
+        a.st->fallback = 1;
+        a.st->ticket = 0;
+    } else {
+        finalize_step(a);
+    }
 }
 
 // Exact max wave speed over the whole new state, run when the candidate max
@@ -1384,6 +1408,12 @@ hipError_t launch_bc_batch(const StepArgs* args, const int* icur, const BatchBlo
     const size_t lds = bc_lds_bytes(img_words, area_slots);
     hipLaunchKernelGGL(lf_bc_ring_batch, dim3((unsigned)nwork), dim3(BC_THREADS), lds, s, args, icur, work, flip, 1,
                        img_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish_batch(const StepArgs* args, const int* icur, int n, int flip, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(lf_finish_batch, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, args, icur, n, flip);
     return hipGetLastError();
 }
 

@@ -51,6 +51,8 @@ hipError_t launch_stencil_batch(const StepArgs* args, const int* icur, const Bat
                                 int flip, bool fast_div, bool buf, double dt_explicit, int loop, hipStream_t s);
 hipError_t launch_bc_batch(const StepArgs* args, const int* icur, const BatchBlock* work, int nwork, int flip,
                            int img_words, int area_slots, hipStream_t s);
+// The step's end (rescan test, dt bookkeeping) of every instance that took it.
+hipError_t launch_finish_batch(const StepArgs* args, const int* icur, int n, int flip, hipStream_t s);
 hipError_t launch_residual_batch(const StepArgs* args, const int* icur, const BatchAux* aux, const BatchBlock* work,
                                  int nwork, int flip, hipStream_t s);
 // Tiles of one instance for tile height H (strips of central columns x row chunks).

@@ -614,10 +614,54 @@ struct mpd_solver {
         checked_step = h.checked_step;
     }
 
+    // Tile height of the tiled stencil when the config leaves it open
+    // (gpu.tile_rows 0).  Large grids: 8 rows (measured fastest at
+    // 6400x2560).  Grids below kTuneCells: a small grid gives few tiles, so a
+    // step is the latency of one tile's march; the heights {1, 2, 4, 8} are
+    // timed once on the current state (explicit dt: the launches write only
+    // the next-state buffer and the row buffer, which a step rewrites before
+    // reading, and the running max, which the loop seed resets) and the
+    // fastest kept.
+    static constexpr double kTuneCells = 4.0e6;
+    bool rows_tuned = false;
+    void TuneTileRows() {
+        rows_tuned = true;
+        if (scfg.kind != mpd::STENCIL_TILED || scfg.tile_rows > 0) return;
+        if ((double)grid.L * grid.Mn >= kTuneCells) return;
+        const StepArgs a = Args(cur);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        int best = 8;
+        float best_ms = 1e30f;
+        for (int h : {8, 4, 2, 1}) {
+            if (h > grid.L) continue;
+            mpd::StencilCfg c = scfg;
+            c.tile_rows = h;
+            StepArgs t = a;
+            t.step1 = 0;
+            HIPCHK(mpd::launch_stencil(t, c, 1.0e-9, 0, 1, stream));
+            HIPCHK(hipEventRecord(e0, stream));
+            for (int r = 0; r < 8; ++r) HIPCHK(mpd::launch_stencil(t, c, 1.0e-9, 0, 1, stream));
+            HIPCHK(hipEventRecord(e1, stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = h;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        scfg.tile_rows = best;
+    }
+
     void Run(int max_steps, mpd_run_result* out) {
         const int step0 = step;
         const bool conv = cfg.convergence_threshold > 0.0;
         if (conv && !prev) throw std::runtime_error("convergence snapshot missing");
+        if (!rows_tuned) TuneTileRows();
         bool done = SeedLoop(max_steps);
         if (!done && step1) {
             EnqueueLoopStep(cur, conv, true);
