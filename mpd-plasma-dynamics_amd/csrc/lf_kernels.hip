@@ -415,16 +415,34 @@ __device__ __forceinline__ void stencil_tile(const StepArgs& a, double dt_explic
 
 template <bool kSpeed, bool kBuf, class Div>
 __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt_explicit, int loop, int H,
-                                                           int nstrips, int ntiles) {
+                                                           int nstrips, int ntiles, int order) {
     if (loop && loop_idle(a)) return;
     // Wave-uniform indices (scalar registers: row addressing and the march
-    // loop then run on the scalar unit).  Tiles in launch order, strip
-    // fastest: the four waves of a workgroup take four adjacent strips of one
-    // band of rows, and the whole GPU works on a narrow band at a time (an
-    // order giving each XCD its own band of rows measured 5% slower).
+    // loop then run on the scalar unit).  Order 0: tiles in launch order,
+    // strip fastest: the four waves of a workgroup take four adjacent strips
+    // of one band of rows, and the whole GPU works on a narrow band at a time
+    // (an order giving each XCD its own contiguous band of rows measured 5%
+    // slower).  Order 1: workgroups go to the XCDs round-robin (workgroup w
+    // to XCD w mod 8), so workgroup w takes chunk 8 q + (w mod 8), strips
+    // 4 g .. 4 g + 3 of it, with (q, g) counting the XCD's own workgroups:
+    // each XCD sweeps whole chunks (all strips of a band of H rows), so the
+    // edge columns two adjacent strips share are read into one L2, and the
+    // XCDs still advance together over a narrow band of rows.
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int tile = (int)blockIdx.x * 4 + wave;
-    if (tile >= ntiles) return;
+    int tile;
+    if (order == 0) {
+        tile = (int)blockIdx.x * 4 + wave;
+        if (tile >= ntiles) return;
+    } else {
+        const int gpc = (nstrips + 3) >> 2;  // workgroups per chunk
+        const int w = (int)blockIdx.x;
+        const int j = w >> 3;
+        const int q = j / gpc, g = j - q * gpc;
+        const int chunk = (q << 3) + (w & 7);
+        const int strip = 4 * g + wave;
+        if (chunk >= ntiles / nstrips || strip >= nstrips) return;
+        tile = chunk * nstrips + strip;
+    }
     stencil_tile<kSpeed, kBuf, Div>(a, dt_explicit, loop, H, nstrips, tile);
 }
 
@@ -1344,20 +1362,21 @@ hipError_t launch_simple(const StepArgs& a, double dt, int loop, int speed, hipS
 }
 
 template <class Div>
-hipError_t launch_tiled(const StepArgs& a, int H, double dt, int loop, int speed, hipStream_t s) {
+hipError_t launch_tiled(const StepArgs& a, int H, int order, double dt, int loop, int speed, hipStream_t s) {
     // Strips holding central columns 2 .. M_max (strip s: columns 62 s .. 62 s + 61).
     const int nstrips = (a.Mn - 1) / kStripCols + 1;
     const int nchunks = (a.L + H - 1) / H;
     const int ntiles = nstrips * nchunks;
-    const unsigned blocks = (unsigned)((ntiles + 3) / 4);
+    const unsigned blocks = order == 0 ? (unsigned)((ntiles + 3) / 4)
+                                       : (unsigned)(8 * ((nchunks + 7) / 8) * ((nstrips + 3) / 4));
     // Buffer addressing needs every row/field offset of a state below 2^31 bytes.
     const bool buf = (double)a.fstride * 8.0 * 8.0 < 2147483648.0;
     if (buf) {
-        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
-        else hipLaunchKernelGGL((lf_stencil_tiled<false, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles, order);
+        else hipLaunchKernelGGL((lf_stencil_tiled<false, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles, order);
     } else {
-        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
-        else hipLaunchKernelGGL((lf_stencil_tiled<false, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles);
+        if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles, order);
+        else hipLaunchKernelGGL((lf_stencil_tiled<false, false, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles, order);
     }
     return hipGetLastError();
 }
@@ -1377,8 +1396,8 @@ hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt, int
         return launch_simple<DivExact>(a, dt, loop, speed, s);
     }
     const int H = c.tile_rows > 0 ? c.tile_rows : auto_rows(a);
-    if (c.fast_div) return launch_tiled<DivConst>(a, H, dt, loop, speed, s);
-    return launch_tiled<DivExact>(a, H, dt, loop, speed, s);
+    if (c.fast_div) return launch_tiled<DivConst>(a, H, c.order, dt, loop, speed, s);
+    return launch_tiled<DivExact>(a, H, c.order, dt, loop, speed, s);
 }
 
 hipError_t launch_bc(const StepArgs& a, double dt, int loop, int speed, hipStream_t s) {

@@ -27,6 +27,7 @@ struct StencilCfg {
     int kind = STENCIL_TILED;
     bool fast_div = true;
     int tile_rows = 0;  // rows per wave of the tiled kernel (0: chosen from the grid)
+    int order = 0;      // tile order: 0 strip-fastest in launch order, 1 row bands per XCD (see lf_stencil_tiled)
 };
 
 // Central update (+ wave-speed epilogue when speed).  loop: read dt and the

@@ -62,6 +62,20 @@ def test_tiled_odd_tile_rows(mpd):
             _compare_state(sol, o, f"tile_rows={h}")
 
 
+@pytest.mark.parametrize("rows", [1, 3, 8])
+def test_tiled_xcd_band_order(mpd, rows):
+    """Tile order 1 (whole row bands per XCD, workgroups past the last chunk
+    or strip idle) gives the same bits, including chunk counts that are not a
+    multiple of 8 and strip counts that are not a multiple of 4."""
+    text = cases.channel_config(37, 300, geometry="condiv", noise=1e-3, max_steps=9, tile_rows=rows)
+    print(f"tile_order=1 tile_rows={rows}", flush=True)
+    o, _ = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        sol.set_option("tile_order", 1)
+        sol.run()
+        _compare_state(sol, o, f"tile_order=1 rows={rows}")
+
+
 def _spike_config(L=60, M=24, steps=80):
     """Straight channel with a one-cell v_z spike at (z, r) = (0.5, 0.5): the
     LF averaging flattens it, so the max wave speed falls by far more than
