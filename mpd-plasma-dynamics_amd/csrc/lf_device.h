@@ -34,7 +34,7 @@ struct StepArgs {
     int bc_words;         // size of the BC program's LDS image in 8-byte words (bc_image_words)
     int bc_area;          // work-area slots per lane of the boundary kernel (bc_area_slots)
     MpdLoopState* st;
-    const MpdBcProgram* bc;
+    const uint64_t* bc;   // the BC program's image (BcImage layout, bc_words words)
 };
 
 // One workgroup's entry in a batched launch's work list: its instance and,
@@ -59,12 +59,52 @@ struct BatchAux {
     int pad;
 };
 
-// The LDS image of a BC program (the boundary kernels' BcImage): a 40-byte
-// header, then the segments and the expression code in use.
+// Image of a BC program, built on the host and staged into LDS by the
+// boundary kernels: the counts, then the segments in use (faces in the order
+// L_LO, L_HI, M_LO, M_HI, each face's in config order), then the expression
+// code in use.  Its size follows the program, so the boundary kernels load
+// exactly bc_image_words words in one round trip and take the image, with
+// the work area behind it, in dynamic LDS.
+struct BcImage {
+    int32_t nseg[4];
+    int32_t ncode;
+    int32_t first[4];  // index of face f's first segment
+    int32_t nsegs;
+    __host__ __device__ __forceinline__ const MpdSeg& seg(int f, int s) const {
+        return reinterpret_cast<const MpdSeg*>(this + 1)[first[f] + s];
+    }
+    __host__ __device__ __forceinline__ const MpdInstr& code(int i) const {
+        return reinterpret_cast<const MpdInstr*>(reinterpret_cast<const MpdSeg*>(this + 1) + nsegs)[i];
+    }
+};
 constexpr int kBcImageHeadWords = 5;
+static_assert(sizeof(BcImage) == 8 * kBcImageHeadWords, "image header layout");
+static_assert(sizeof(MpdSeg) % 8 == 0 && sizeof(MpdInstr) == 16, "image word layout");
+
 __host__ __device__ inline int bc_image_words(const MpdBcProgram& p) {
     return kBcImageHeadWords + (int)(sizeof(MpdSeg) / 8) * (p.nseg[0] + p.nseg[1] + p.nseg[2] + p.nseg[3]) +
            2 * p.ncode;
+}
+
+// Writes the image of p into out (bc_image_words(p) words).
+inline void build_bc_image(const MpdBcProgram& p, uint64_t* out) {
+    BcImage h{};
+    int first = 0;
+    for (int f = 0; f < 4; ++f) {
+        h.nseg[f] = p.nseg[f];
+        h.first[f] = first;
+        first += p.nseg[f];
+    }
+    h.ncode = p.ncode;
+    h.nsegs = first;
+    char* o = reinterpret_cast<char*>(out);
+    __builtin_memcpy(o, &h, sizeof(h));
+    o += sizeof(h);
+    for (int f = 0; f < 4; ++f) {
+        __builtin_memcpy(o, &p.seg[f][0], sizeof(MpdSeg) * p.nseg[f]);
+        o += sizeof(MpdSeg) * p.nseg[f];
+    }
+    __builtin_memcpy(o, &p.code[0], sizeof(MpdInstr) * p.ncode);
 }
 
 // Per-lane work area of the boundary kernel, in slots of one double: the

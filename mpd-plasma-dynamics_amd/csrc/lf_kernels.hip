@@ -499,26 +499,6 @@ struct WorkArea {
     }
 };
 
-// LDS image of a BC program, staged once per workgroup: the counts, then the
-// segments in use (faces in the order L_LO, L_HI, M_LO, M_HI, each face's in
-// config order), then the expression code in use.  Its size follows the
-// program (StepArgs::bc_words), so the boundary kernels take it, with the
-// work area behind it, in dynamic LDS: a sweep's small programs leave room
-// for more workgroups per CU than the largest program would.
-struct BcImage {
-    int32_t nseg[4];
-    int32_t ncode;
-    int32_t first[4];  // index of face f's first segment
-    int32_t nsegs;
-    __device__ __forceinline__ const MpdSeg& seg(int f, int s) const {
-        return reinterpret_cast<const MpdSeg*>(this + 1)[first[f] + s];
-    }
-    __device__ __forceinline__ const MpdInstr& code(int i) const {
-        return reinterpret_cast<const MpdInstr*>(reinterpret_cast<const MpdSeg*>(this + 1) + nsegs)[i];
-    }
-};
-static_assert(sizeof(BcImage) == 8 * kBcImageHeadWords, "image header layout");
-
 // The staged image a boundary kernel's StepArgs points to.
 __device__ __forceinline__ const BcImage* img(const StepArgs& a) { return reinterpret_cast<const BcImage*>(a.bc); }
 
@@ -873,61 +853,23 @@ __device__ void finalize_step(const StepArgs& a) {
     st->ticket = 0;
 }
 
-// Builds the LDS image of the BC program (BcImage) in `prog`: the header
-// from the counts (read first), then each thread issues all its loads of the
-// segments and code in use before its LDS stores, so the copy costs two
-// global round trips (a plain strided copy loop waits for each load in turn:
-// 10 us measured at 64 threads).
-__device__ __forceinline__ void stage_program(const MpdBcProgram* bc, uint64_t* prog) {
-    constexpr int kHead = (int)(offsetof(MpdBcProgram, seg) / 8);
-    constexpr int kSegW = (int)(sizeof(MpdSeg) / 8);
-    constexpr int kFaceW = kSegW * MPD_MAX_SEGS_PER_FACE;
-    constexpr int kCode = (int)(offsetof(MpdBcProgram, code) / 8);
-    static_assert(offsetof(MpdBcProgram, seg) % 8 == 0 && sizeof(MpdSeg) % 8 == 0 && sizeof(MpdInstr) == 16, "");
-    const uint64_t* src = reinterpret_cast<const uint64_t*>(bc);
-    const int n0 = bc->nseg[0], n1 = bc->nseg[1], n2 = bc->nseg[2], n3 = bc->nseg[3];
-    const int ncode = bc->ncode;
-    if (threadIdx.x == 0) {
-        BcImage* h = reinterpret_cast<BcImage*>(prog);
-        h->nseg[0] = n0;
-        h->nseg[1] = n1;
-        h->nseg[2] = n2;
-        h->nseg[3] = n3;
-        h->ncode = ncode;
-        h->first[0] = 0;
-        h->first[1] = n0;
-        h->first[2] = n0 + n1;
-        h->first[3] = n0 + n1 + n2;
-        h->nsegs = n0 + n1 + n2 + n3;
-    }
-    const int w0 = kSegW * n0;
-    const int w1 = w0 + kSegW * n1;
-    const int w2 = w1 + kSegW * n2;
-    const int w3 = w2 + kSegW * n3;
-    const int total = w3 + 2 * ncode;
-    uint64_t* dst = prog + kBcImageHeadWords;
+// Copies the program image (words 8-byte words) into `prog` (LDS).  Every
+// thread issues all its loads before its LDS stores: one global round trip
+// per 512 words (a plain strided copy loop waits for each load in turn).
+__device__ __forceinline__ void stage_image(const uint64_t* src, uint64_t* prog, int words) {
     constexpr int kPer = 8;
-    for (int base = 0; base < total; base += kPer * BC_THREADS) {
+    for (int base = 0; base < words; base += kPer * BC_THREADS) {
         uint64_t v[kPer];
-        int d[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             const int i = base + j * BC_THREADS + (int)threadIdx.x;
-            d[j] = -1;
-            if (i < total) {
-                int o;
-                if (i < w0) o = kHead + i;
-                else if (i < w1) o = kHead + kFaceW + (i - w0);
-                else if (i < w2) o = kHead + 2 * kFaceW + (i - w1);
-                else if (i < w3) o = kHead + 3 * kFaceW + (i - w2);
-                else o = kCode + (i - w3);
-                d[j] = i;
-                v[j] = src[o];
-            }
+            v[j] = (i < words) ? src[i] : 0;
         }
 #pragma unroll
-        for (int j = 0; j < kPer; ++j)
-            if (d[j] >= 0) dst[d[j]] = v[j];
+        for (int j = 0; j < kPer; ++j) {
+            const int i = base + j * BC_THREADS + (int)threadIdx.x;
+            if (i < words) prog[i] = v[j];
+        }
     }
     __syncthreads();
 }
@@ -939,12 +881,10 @@ __device__ __forceinline__ void stage_program(const MpdBcProgram* bc, uint64_t* 
 // area are the block's LDS.  finish: the last block to complete finishes the
 // loop step (ticket and device-scope fences); the batched path finishes its
 // instances' steps in a kernel of its own instead (lf_finish_batch).
-__device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt_explicit, int loop, int speed, int bid,
-                                              int nblk, uint64_t* prog, double* area, bool finish) {
-    const double dt = step_dt(a_in, dt_explicit, loop);
-    stage_program(a_in.bc, prog);
+__device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt, int loop, int speed, int bid,
+                                              int nblk, const uint64_t* prog, double* area, bool finish) {
     StepArgs a = a_in;
-    a.bc = reinterpret_cast<const MpdBcProgram*>(prog);  // the image, read through img(a)
+    a.bc = prog;  // the staged image, read through img(a)
     const WorkArea w{area, (int)threadIdx.x};
     const int L = a.L, Mn = a.Mn;
     const int nring = 2 * Mn + 2 * (L - 2);
@@ -1020,12 +960,17 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt_ex
 }
 
 __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double dt_explicit, int loop, int speed) {
-    if (loop && loop_idle(a_in)) return;
     // The BC program (segments and the expression code in use) is read by
     // every thread many times over in dependent chains: its image goes to
-    // LDS once, the per-lane work area behind it.
+    // LDS once, the per-lane work area behind it.  The loop-state reads and
+    // the image loads are issued together (one round trip), the idle test
+    // taken after.
     extern __shared__ uint64_t bc_lds[];
-    bc_ring_block(a_in, dt_explicit, loop, speed, (int)blockIdx.x, (int)gridDim.x, bc_lds,
+    const bool idle = loop && loop_idle(a_in);
+    const double dt = step_dt(a_in, dt_explicit, loop);
+    stage_image(a_in.bc, bc_lds, a_in.bc_words);
+    if (idle) return;
+    bc_ring_block(a_in, dt, loop, speed, (int)blockIdx.x, (int)gridDim.x, bc_lds,
                   reinterpret_cast<double*>(bc_lds + a_in.bc_words), true);
 }
 
@@ -1038,10 +983,14 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring_batch(const StepArgs* _
                                                                int speed, int img_words) {
     const BatchBlock w = work[blockIdx.x];
     const StepArgs a = args[2 * w.inst + (icur[w.inst] ^ flip)];
-    if (loop_idle(a)) return;
-    // img_words: the largest image of the launch's instances.
+    // img_words: the largest image of the launch's instances (the work area's
+    // offset); this instance's image is a.bc_words long.
     extern __shared__ uint64_t bc_lds[];
-    bc_ring_block(a, 0.0, 1, speed, w.first, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words), false);
+    const bool idle = loop_idle(a);
+    const double dt = step_dt(a, 0.0, 1);
+    stage_image(a.bc, bc_lds, a.bc_words);
+    if (idle) return;
+    bc_ring_block(a, dt, 1, speed, w.first, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words), false);
 }
 
 // The step's end for every instance of a batch that took it (after the

@@ -198,7 +198,7 @@ struct mpd_solver {
     double* rowsum = nullptr;
     double* scratch = nullptr;
     double* geo = nullptr;
-    MpdBcProgram* bc = nullptr;
+    uint64_t* bc = nullptr;  // the BC program's image (BcImage layout)
     MpdLoopState* st = nullptr;
     unsigned long long* umax = nullptr;
     hipStream_t stream = nullptr;
@@ -397,9 +397,13 @@ struct mpd_solver {
         HIPCHK(hipMemcpy(geo, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(mpd::launch_recip_table(geo + 4 * rows, geo + 5 * rows, rows, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        HIPCHK(hipMalloc((void**)&bc, sizeof(MpdBcProgram)));
-        HIPCHK(hipMemcpy(bc, &hbc, sizeof(MpdBcProgram), hipMemcpyHostToDevice));
         bc_words = mpd::bc_image_words(hbc);
+        {
+            std::vector<uint64_t> image((size_t)bc_words);
+            mpd::build_bc_image(hbc, image.data());
+            HIPCHK(hipMalloc((void**)&bc, image.size() * sizeof(uint64_t)));
+            HIPCHK(hipMemcpy(bc, image.data(), image.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+        }
         bc_area = mpd::bc_area_slots(hbc);
         if (ext_st) {
             st = ext_st;
