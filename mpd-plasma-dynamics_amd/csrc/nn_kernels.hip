@@ -273,13 +273,24 @@ __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
             }
 }
 
-// dw[i] = sum over splits s (in order) of part[s][i].
-__global__ void reduce_splits(const float* part, float* dw, int64_t n, int S) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int z = 0; z < S; ++z) s += part[(int64_t)z * n + i];
-        dw[i] = s;
+// out[i] = sum over rows s < S of part[s][i], i < n: a block of 16 waves
+// per 64 outputs, wave w summing rows w, w + 16, ..., then the 16 partial
+// sums added in a fixed tree order (deterministic, whatever the hardware
+// schedule).
+__global__ void __launch_bounds__(1024) column_sum(const float* part, float* out, int64_t n, int S) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int w = (int)(threadIdx.x >> 6);
+    float s = 0.f;
+    if (i < n)
+        for (int z = w; z < S; z += 16) s += part[(int64_t)z * n + i];
+    __shared__ float sh[16][64];
+    sh[w][threadIdx.x & 63] = s;
+    __syncthreads();
+    for (int h = 8; h > 0; h >>= 1) {
+        if (w < h) sh[w][threadIdx.x & 63] += sh[w + h][threadIdx.x & 63];
+        __syncthreads();
     }
+    if (w == 0 && i < n) out[i] = sh[0][threadIdx.x];
 }
 
 // Bias gradient, pass 1: block (cb, s) sums channels 64 cb .. 64 cb + 63
@@ -297,20 +308,6 @@ __global__ void __launch_bounds__(256) bias_grad_part(const float* dz, float* pa
     __syncthreads();
     if (w == 0 && c < C) part[(int64_t)blockIdx.y * C + c] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) +
                                                              (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
-}
-
-// Pass 2: db[c] = sum over slices (the 4 waves take interleaved slices,
-// then a fixed-order combine: deterministic).
-__global__ void __launch_bounds__(256) bias_grad_sum(const float* part, float* db, int C, int S) {
-    const int c = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
-    const int w = (int)(threadIdx.x >> 6);
-    float s = 0.f;
-    if (c < C)
-        for (int k = w; k < S; k += 4) s += part[(int64_t)k * C + c];
-    __shared__ float sh[4][64];
-    sh[w][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (w == 0 && c < C) db[c] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
 }
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
@@ -499,10 +496,10 @@ bool conv_dims_ok(int B, int H, int W, int Cin, int Cout) {
            (int64_t)9 * Cin < (1 << 20) && (int64_t)9 * Cout < (1 << 20);
 }
 
-// Pixel slices of the bias gradient: about 512 pixels each, at most 256.
+// Pixel slices of the bias gradient: about 128 pixels each, at most 1024.
 int bias_slices(int P, int C) {
     (void)C;
-    return std::max(1, std::min(256, P / 512));
+    return std::max(1, std::min(1024, P / 128));
 }
 
 // Pixel splits of the weight gradient: enough workgroups to fill the GPU
@@ -605,7 +602,7 @@ int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, 
     if (Cin % 8 == 0 && Cout % 8 == 0) hipLaunchKernelGGL((conv_gemm<MODE_WGRAD, true>), g, dim3(256), 0, s, a);
     else hipLaunchKernelGGL((conv_gemm<MODE_WGRAD, false>), g, dim3(256), 0, s, a);
     const int64_t n = (int64_t)a.M * a.N;
-    hipLaunchKernelGGL(reduce_splits, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)work, dw, n, S);
+    hipLaunchKernelGGL(column_sum, dim3((unsigned)((n + 63) / 64)), dim3(1024), 0, s, (const float*)work, dw, n, S);
     return finish(hipSuccess);
 }
 
@@ -616,7 +613,7 @@ int mpd_nn_bias_grad(const float* dz, float* db, float* work, int P, int C, void
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(bias_grad_part, dim3((unsigned)((C + 63) / 64), (unsigned)S), dim3(256), 0, s, dz, work, P, C,
                        rows);
-    hipLaunchKernelGGL(bias_grad_sum, dim3((unsigned)((C + 63) / 64)), dim3(256), 0, s, (const float*)work, db, C, S);
+    hipLaunchKernelGGL(column_sum, dim3((unsigned)((C + 63) / 64)), dim3(1024), 0, s, (const float*)work, db, (int64_t)C, S);
     return finish(hipSuccess);
 }
 
