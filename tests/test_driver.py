@@ -126,3 +126,43 @@ def test_driver_thruster_end_to_end(tmp_path):
     assert np.array_equal(arr["Energy"][:, :, 0], g(4))
     assert np.array_equal(arr["Velocity"], np.stack([g(1), g(2), g(3)], axis=-1))
     assert np.array_equal(arr["MagneticField"], np.stack([g(7), g(8), g(9)], axis=-1))
+
+
+SWEEP = os.path.join(REPO, "mpd-plasma-dynamics_amd", "mpd_sweep_run")
+
+
+@pytest.mark.gpu
+def test_sweep_driver_cases_match_oracle(tmp_path):
+    """mpd_sweep_run on three configs of different size and boundary program
+    (one stops on convergence, two on their step caps): every case's printed
+    loop end and outlet diagnostics, and its final VTK frame, are the
+    oracle's run of that config alone."""
+    assert os.path.exists(SWEEP), "build mpd_sweep_run first"
+    texts = [cases.channel_config(40, 16, max_steps=60),
+             cases.thruster_config(48, 24, max_steps=40),
+             cases.channel_config(30, 12, max_steps=5000, rho="1.3", threshold=2e-3, check_frequency=10)]
+    names = []
+    for i, t in enumerate(texts):
+        (tmp_path / f"c{i}.yaml").write_text(t)
+        names.append(f"c{i}.yaml")
+    print("sweep driver: 3 cases", flush=True)
+    p = subprocess.run([SWEEP, "--out", "frames", *names], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    out = p.stdout.splitlines()
+    assert out[0] == "[sweep] 3 cases"
+    rows = {int(l.split()[0]): l.split() for l in out if re.match(r"^\d+ +c\d\.yaml ", l)}
+    assert sorted(rows) == [0, 1, 2]
+    for i, t in enumerate(texts):
+        case = cases.oracle_case(t)
+        o = O.Oracle(case)
+        res = o.run(case["max_steps"])
+        r = rows[i]
+        assert r[1] == names[i] and r[2] == f"{case['L']}x{case['M']}"
+        assert int(r[3]) == res["steps"], (i, r, res)
+        assert _close(r[4], res["t"], 1e-6) and _close(r[5], res["dt"], 1e-6)
+        assert r[7] == ("yes" if res["converged"] else "no")
+        assert _close(r[8], o.mass_flux(), 1e-6) and _close(r[9], o.thrust(), 1e-6)
+        _, ni, nj, _, arr = _parse_legacy(tmp_path / "frames" / f"case_{i:04d}.vtk")
+        li = np.minimum(np.arange(ni), case["L"] - 1)
+        assert np.array_equal(arr["Rho"][:, :, 0], o.phys(0)[li + 1][:, 1:case["M"] + 2].T)
+    assert rows[2][7] == "yes" and rows[0][7] == "no"
