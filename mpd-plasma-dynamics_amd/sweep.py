@@ -136,17 +136,81 @@ def probe_configs(kind, seed=7):
     return cfgs
 
 
+def converge_uneven(mpd_amd, sizes, n=64, seed=2, threshold=1e-8, plain_sample=16, chunk=2000):
+    """n seeded cases with nz drawn uniformly from `sizes` (nr = 0.4 nz, the
+    parameter box of sample_params), every case run to SolutionChange <
+    threshold: once batched, once one case at a time (plain_sample cases
+    timed, scaled to n by the mean of each size).  Returns a dict."""
+    import json  # noqa: F401  (the caller prints)
+    rng = np.random.default_rng(seed)
+    nz = rng.choice(np.asarray(sizes), size=n)
+    params = sample_params(n, seed)
+    cfgs = [case_config(params[i], int(nz[i]), max(2, int(round(0.4 * nz[i]))), threshold=threshold,
+                        max_steps=3000000) for i in range(n)]
+    cells = sum(int(z) * (max(2, int(round(0.4 * z))) + 1) for z in nz)
+    finish = np.full(n, np.nan)
+    steps = np.zeros(n, int)
+    with mpd_amd.Batch(cfgs) as b:
+        b.autotune(5)
+        t0 = time.perf_counter()
+        last = 0.0
+        while True:
+            res = b.run(chunk)
+            el = time.perf_counter() - t0
+            steps += np.array([r["steps"] for r in res])
+            for i, r in enumerate(res):
+                if r["done"] and np.isnan(finish[i]):
+                    finish[i] = el
+            nd = sum(r["done"] for r in res)
+            if el - last >= 10.0 or nd == n:
+                print(f"[sweep] batched: {el:.1f} s, {nd}/{n} done, max steps {steps.max()}", flush=True)
+                last = el
+            if nd == n:
+                break
+        batch_s = time.perf_counter() - t0
+        conv = all(r["converged"] for r in res)
+    # One case at a time: the same configs, sampled evenly over the sizes.
+    per_size = {}
+    order = np.argsort(nz, kind="stable")
+    pick = order[np.linspace(0, n - 1, min(n, plain_sample)).astype(int)]
+    for i in pick:
+        with mpd_amd.Solver(cfgs[i]) as sol:
+            t1 = time.perf_counter()
+            r = sol.run()
+            per_size.setdefault(int(nz[i]), []).append(time.perf_counter() - t1)
+            assert r["converged"]
+        print(f"[sweep] one case at a time: nz {nz[i]}, {r['steps']} steps, {per_size[int(nz[i])][-1]:.2f} s",
+              flush=True)
+    mean = {z: float(np.mean(v)) for z, v in per_size.items()}
+    plain_s = float(sum(mean[int(z)] for z in nz))
+    return {"cases": n, "sizes": [int(z) for z in sizes], "counts": [int(np.sum(nz == z)) for z in sizes],
+            "cells": int(cells), "threshold": threshold, "all_converged": bool(conv),
+            "batched_s": batch_s, "one_at_a_time_s": plain_s, "one_at_a_time_timed": int(len(pick)),
+            "one_at_a_time_mean_s_by_size": mean, "speedup": plain_s / batch_s,
+            "steps_p50": float(np.median(steps)), "steps_max": int(steps.max()),
+            "finish_p50_s": float(np.nanmedian(finish)), "finish_p99_s": float(np.nanpercentile(finish, 99))}
+
+
 def main():
     import argparse
+    import json
     import os
     import sys
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import mpd_amd
     ap = argparse.ArgumentParser(description="time a batched sweep workload")
-    ap.add_argument("--probe", choices=["small", "uneven"], required=True)
+    ap.add_argument("--probe", choices=["small", "uneven"])
+    ap.add_argument("--converge", type=str, default="",
+                    help="comma-separated nz sizes: run 64 uneven cases to residual 1e-8, batched and one at a time")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     args = ap.parse_args()
+    if args.converge:
+        r = converge_uneven(mpd_amd, [int(v) for v in args.converge.split(",")])
+        print(json.dumps(r, indent=1), flush=True)
+        return
+    if not args.probe:
+        ap.error("--probe or --converge is required")
     with mpd_amd.Batch(probe_configs(args.probe)) as b:
         b.run(args.warmup)
         print(b.autotune(5), flush=True)
