@@ -80,9 +80,15 @@ int mpd_nn_mse_blocks(long long n);
 int mpd_nn_mse(const float* y, const float* t, float* dy, float* part, long long n, void* stream);
 
 /* Adam step (bias-corrected, step >= 1) on FP32 master weights w with
- * moments m, v; w_bf16 (may be NULL) receives the bf16 copy. */
+ * moments m, v, using the gradient g scaled by gscale (global-norm clipping:
+ * 1 leaves it unchanged); w_bf16 (may be NULL) receives the bf16 copy. */
 int mpd_nn_adam(float* w, uint16_t* w_bf16, const float* g, float* m, float* v, long long n, float lr, float b1,
-                float b2, float eps, int step, void* stream);
+                float b2, float eps, int step, float gscale, void* stream);
+
+/* mpd_nn_sumsq_blocks(n) partial sums of x[i]^2 in part (the caller adds
+ * them in order: a deterministic squared norm). */
+int mpd_nn_sumsq_blocks(long long n);
+int mpd_nn_sumsq(const float* x, float* part, long long n, void* stream);
 
 /* dst[j] = src[idx[j]] for rows of row_len floats (mini-batch gather). */
 int mpd_nn_gather_rows(const float* src, const int* idx, float* dst, int nrows, long long row_len, void* stream);

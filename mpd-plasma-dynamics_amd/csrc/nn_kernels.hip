@@ -451,11 +451,12 @@ __global__ void __launch_bounds__(256) mse(const float* y, const float* t, float
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// Adam (Kingma & Ba 2015) with bias correction; writes the bf16 copy.
+// Adam (Kingma & Ba 2015) with bias correction on the gradient scaled by
+// gscale (global-norm clipping); writes the bf16 copy.
 __global__ void adam(float* w, uint16_t* wb, const float* g, float* m, float* v, int64_t n, float lr, float b1,
-                     float b2, float eps, float c1, float c2) {
+                     float b2, float eps, float c1, float c2, float gscale) {
     GRID_STRIDE(i, n) {
-        const float gi = g[i];
+        const float gi = g[i] * gscale;
         const float mi = b1 * m[i] + (1.f - b1) * gi;
         const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
         m[i] = mi;
@@ -464,6 +465,20 @@ __global__ void adam(float* w, uint16_t* wb, const float* g, float* m, float* v,
         w[i] = wi;
         if (wb) wb[i] = f2bf(wi);
     }
+}
+
+// Per-block partial sums of x[i]^2 (part[blockIdx.x]; fixed tree order).
+__global__ void __launch_bounds__(256) sumsq(const float* x, float* part, int64_t n) {
+    float s = 0.f;
+    GRID_STRIDE(i, n) s += x[i] * x[i];
+    __shared__ float sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // dst[j] = src[idx[j]] (rows of row_len floats).
@@ -700,11 +715,20 @@ int mpd_nn_mse(const float* y, const float* t, float* dy, float* part, long long
 }
 
 int mpd_nn_adam(float* w, uint16_t* w_bf16, const float* g, float* m, float* v, long long n, float lr, float b1,
-                float b2, float eps, int step, void* stream) {
+                float b2, float eps, int step, float gscale, void* stream) {
     if (!w || !g || !m || !v || n < 0 || step < 1) return bad("adam: bad argument");
     const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
     hipLaunchKernelGGL(adam, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, w, w_bf16, g, m, v, (int64_t)n,
-                       lr, b1, b2, eps, c1, c2);
+                       lr, b1, b2, eps, c1, c2, gscale);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_sumsq_blocks(long long n) { return (int)std::min<long long>(64, std::max<long long>(1, (n + 255) / 256)); }
+
+int mpd_nn_sumsq(const float* x, float* part, long long n, void* stream) {
+    if (!x || !part || n <= 0) return bad("sumsq: bad argument");
+    hipLaunchKernelGGL(sumsq, dim3((unsigned)mpd_nn_sumsq_blocks(n)), dim3(256), 0, (hipStream_t)stream, x, part,
+                       (int64_t)n);
     return finish(hipSuccess);
 }
 

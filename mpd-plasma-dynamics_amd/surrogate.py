@@ -69,7 +69,9 @@ def nn_lib():
         "mpd_nn_add": [_vp, _vp, ll, _vp],
         "mpd_nn_mse_blocks": [ll],
         "mpd_nn_mse": [_vp, _vp, _vp, _vp, ll, _vp],
-        "mpd_nn_adam": [_vp, _vp, _vp, _vp, _vp, ll, f, f, f, f, i, _vp],
+        "mpd_nn_adam": [_vp, _vp, _vp, _vp, _vp, ll, f, f, f, f, i, f, _vp],
+        "mpd_nn_sumsq_blocks": [ll],
+        "mpd_nn_sumsq": [_vp, _vp, ll, _vp],
         "mpd_nn_gather_rows": [_vp, _vp, _vp, i, ll, _vp],
         "mpd_nn_malloc": [C.POINTER(_vp), ll],
         "mpd_nn_free": [_vp],
@@ -92,7 +94,7 @@ NN_EXPORTS = ["mpd_nn_last_error", "mpd_nn_conv3x3_fwd", "mpd_nn_conv3x3_bwd_dat
               "mpd_nn_conv3x3_bwd_weight", "mpd_nn_bias_work_floats", "mpd_nn_bias_grad", "mpd_nn_silu_fwd", "mpd_nn_silu_bwd",
               "mpd_nn_to_bf16", "mpd_nn_avgpool2_fwd", "mpd_nn_avgpool2_bwd", "mpd_nn_upsample2_fwd",
               "mpd_nn_upsample2_bwd", "mpd_nn_concat_fwd", "mpd_nn_concat_bwd", "mpd_nn_add", "mpd_nn_mse_blocks",
-              "mpd_nn_mse", "mpd_nn_adam", "mpd_nn_gather_rows", "mpd_nn_malloc", "mpd_nn_free",
+              "mpd_nn_mse", "mpd_nn_adam", "mpd_nn_sumsq_blocks", "mpd_nn_sumsq", "mpd_nn_gather_rows", "mpd_nn_malloc", "mpd_nn_free",
               "mpd_nn_memcpy_h2d", "mpd_nn_memcpy_d2h", "mpd_nn_memset", "mpd_nn_synchronize"]
 
 
@@ -235,6 +237,18 @@ class UNet:
             self.w[k].upload(ws[k])
             self.b[k].upload(bs[k])
         self._sync_bf16()
+
+    def snapshot(self):
+        """Host copies of the FP32 weights and biases."""
+        return [w.download() for w in self.w], [b.download() for b in self.b]
+
+    def restore(self, state):
+        """Weights and biases from a snapshot; Adam starts afresh (zero moments,
+        step count 0)."""
+        self.load_params(*state)
+        for t in self.mw + self.vw + self.mb + self.vb:
+            t.zero()
+        self.step = 0
 
     def n_params(self):
         return sum(w.numel() + b.numel() for w, b in zip(self.w, self.b))
@@ -394,14 +408,39 @@ class UNet:
             k -= 1
         assert k == -1
 
-    def adam_step(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+    def grad_norm(self):
+        """Global L2 norm of all parameter gradients (partial sums on the GPU,
+        added on the host in a fixed order)."""
+        L, s = nn_lib(), _stream()
+        if getattr(self, "_gparts", None) is None:
+            self._gslots = []
+            off = 0
+            for t in self.gw + self.gb:
+                nb = L.mpd_nn_sumsq_blocks(t.numel())
+                self._gslots.append((off, nb))
+                off += nb
+            self._gparts = DeviceArray(off)
+        for t, (off, nb) in zip(self.gw + self.gb, self._gslots):
+            _ck(L.mpd_nn_sumsq(_p(t), C.c_void_p(self._gparts.ptr + 4 * off), t.numel(), s))
+        return float(np.sqrt(self._gparts.download().astype(np.float64).sum()))
+
+    def adam_step(self, lr, b1=0.9, b2=0.999, eps=1e-8, clip=None):
+        """One Adam step; with clip, gradients whose global norm exceeds clip
+        are scaled down to it (returns the norm, or None without clip)."""
         L, s = nn_lib(), _stream()
         self.step += 1
+        norm = None
+        gscale = 1.0
+        if clip is not None:
+            norm = self.grad_norm()
+            if norm > clip:
+                gscale = clip / norm
         for k in range(len(self.layers)):
             _ck(L.mpd_nn_adam(_p(self.w[k]), _p(self.wb[k]), _p(self.gw[k]), _p(self.mw[k]), _p(self.vw[k]),
-                              self.w[k].numel(), lr, b1, b2, eps, self.step, s))
+                              self.w[k].numel(), lr, b1, b2, eps, self.step, gscale, s))
             _ck(L.mpd_nn_adam(_p(self.b[k]), None, _p(self.gb[k]), _p(self.mb[k]), _p(self.vb[k]),
-                              self.b[k].numel(), lr, b1, b2, eps, self.step, s))
+                              self.b[k].numel(), lr, b1, b2, eps, self.step, gscale, s))
+        return norm
 
     def mse(self, y, t, dy):
         """Mean squared error (host float) and dL/dy into dy (may be None)."""

@@ -122,10 +122,16 @@ class Normalizer:
         return {"mean": self.mean.tolist(), "std": self.std.tolist()}
 
 
-def train(net, X, T, epochs, batch, lr, seed=0, log_every=20):
+def train(net, X, T, epochs, batch, lr, seed=0, log_every=20, clip=None, guard=3.0, stats=None):
     """Adam on the GPU kernels.  X: (N, H, W, 8) float32 features, T: (N, H,
     W, 8) float32 normalized targets (host arrays, uploaded once).  Cosine
-    learning-rate decay.  Returns the per-epoch mean losses."""
+    learning-rate decay.  Divergence guard: the weights of the best epoch so
+    far are kept; an epoch whose mean loss exceeds guard x the best restores
+    them, restarts Adam and halves the learning rate (a late gradient spike
+    against Adam's small second moments can otherwise leave the network
+    stuck at a constant output).  clip: optional global gradient-norm clip.
+    Returns the per-epoch mean losses; `stats` (a dict) receives the number
+    of clipped steps, the largest gradient norm and the rollbacks."""
     N, H, W, C = X.shape
     Xd = SG.DeviceArray.from_numpy(X.reshape(N, -1))
     Td = SG.DeviceArray.from_numpy(T.reshape(N, -1))
@@ -140,6 +146,8 @@ def train(net, X, T, epochs, batch, lr, seed=0, log_every=20):
     total = epochs * nb
     losses = []
     it = 0
+    clipped, gmax = 0, 0.0
+    best, best_state, lr_scale, rollbacks = np.inf, None, 1.0, 0
     t0 = time.perf_counter()
     for ep in range(epochs):
         perm = rng.permutation(N)
@@ -152,12 +160,30 @@ def train(net, X, T, epochs, batch, lr, seed=0, log_every=20):
             y = net.forward(xb16, batch)
             acc += net.mse(y, tb, dy)
             net.backward(dy, batch)
-            lr_t = 0.5 * lr * (1 + np.cos(np.pi * it / total))
-            net.adam_step(float(lr_t))
+            lr_t = 0.5 * lr * lr_scale * (1 + np.cos(np.pi * it / total))
+            g = net.adam_step(float(lr_t), clip=clip)
+            if g is not None:
+                gmax = max(gmax, g)
+                clipped += int(g > clip)
             it += 1
         losses.append(acc / nb)
+        if guard and best_state is not None and losses[-1] > guard * best:
+            net.restore(best_state)
+            lr_scale *= 0.5
+            rollbacks += 1
+            progress(f"epoch {ep}: loss {losses[-1]:.4g} > {guard} x best {best:.4g}: back to the best weights, "
+                     f"learning rate x {lr_scale}")
+        elif losses[-1] < best:
+            best = losses[-1]
+            if guard:
+                best_state = net.snapshot()
         if ep % log_every == 0 or ep == epochs - 1:
-            progress(f"epoch {ep}: loss {losses[-1]:.4g} ({time.perf_counter() - t0:.1f}s)")
+            extra = f", {clipped} clipped steps, largest gradient norm {gmax:.3g}" if clip is not None else ""
+            progress(f"epoch {ep}: loss {losses[-1]:.4g} ({time.perf_counter() - t0:.1f}s{extra})")
+    if guard and best_state is not None and losses[-1] > guard * best:
+        net.restore(best_state)
+    if stats is not None:
+        stats.update(clipped_steps=clipped, max_grad_norm=gmax, steps=it, rollbacks=rollbacks)
     return losses
 
 
@@ -326,6 +352,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--clip", type=float, default=None, help="optional global gradient-norm clip")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default="")
@@ -346,11 +373,13 @@ def main():
     net = SG.UNet(L, M + 1, seed=args.seed)
     progress(f"network: {net.n_params()} parameters, {len(net.layers)} convs")
     t1 = time.perf_counter()
-    losses = train(net, X, T, args.epochs, args.batch, args.lr, seed=args.seed)
+    tstats = {}
+    losses = train(net, X, T, args.epochs, args.batch, args.lr, seed=args.seed, clip=args.clip, stats=tstats)
     train_s = time.perf_counter() - t1
     test_params = sweep.sample_params(args.test, args.seed + 1)
     result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=args.plain_sample)
     result.update({"train_cases": int(keep.sum()), "generate_s": gen_s, "train_s": train_s, "epochs": args.epochs,
+                   "clip": args.clip, "rollbacks": tstats["rollbacks"],
                    "final_loss": losses[-1], "first_loss": losses[0], "n_params": net.n_params()})
     print(json.dumps(result, indent=1), flush=True)
     if args.out:

@@ -146,10 +146,11 @@ def test_elementwise_kernels_match_oracle(nn):
     V.zero()
     WB = _empty((50,), bf16=True)
     for step in (1, 2):
-        nn._ck(L.mpd_nn_adam(nn._p(Wd), nn._p(WB), nn._p(G), nn._p(M), nn._p(V), 50, 1e-2, 0.9, 0.999, 1e-8, step, s))
+        nn._ck(L.mpd_nn_adam(nn._p(Wd), nn._p(WB), nn._p(G), nn._p(M), nn._p(V), 50, 1e-2, 0.9, 0.999, 1e-8, step,
+                             0.5, s))
     rw, rm, rv = w0, np.zeros(50), np.zeros(50)
     for step in (1, 2):
-        rw, rm, rv = S.adam(rw, g, rm, rv, 1e-2, 0.9, 0.999, 1e-8, step)
+        rw, rm, rv = S.adam(rw, 0.5 * g, rm, rv, 1e-2, 0.9, 0.999, 1e-8, step)
     np.testing.assert_allclose(_host(Wd), rw, rtol=1e-5, atol=1e-6)
     assert np.array_equal(_host_bf16(WB), S.bf16_round(_host(Wd)))
 
@@ -161,6 +162,12 @@ def test_elementwise_kernels_match_oracle(nn):
         ref = dzb.astype(np.float64).sum(axis=0)
         mag = np.abs(dzb.astype(np.float64)).sum(axis=0)
         assert np.all(np.abs(_host(DB) - ref) <= P * 2.0 ** -23 * mag)
+
+    xs = rng.normal(size=70000).astype(np.float32)
+    part = _empty((L.mpd_nn_sumsq_blocks(xs.size),))
+    nn._ck(L.mpd_nn_sumsq(nn._p(_dev_f32(xs)), nn._p(part), xs.size, s))
+    ref = np.sum(xs.astype(np.float64) ** 2)
+    assert abs(_host(part).sum() - ref) <= xs.size * 2.0 ** -23 * ref
 
     src = rng.normal(size=(6, 4)).astype(np.float32)
     idx = np.array([5, 0, 3], np.int32)
