@@ -85,7 +85,10 @@ def steady_fields(b, n, L, M):
 
 def generate(n, L, M, seed, threshold=1e-8, max_steps=400000, batch=512):
     """n seeded sweep cases solved cold to the threshold by the batched
-    solver.  Returns (params, fields (n, L, Mn, 8), steps, converged)."""
+    solver, up to `batch` cases per batch (each case's result is the same
+    whatever it is batched with; one large batch pays the latency-bound tail
+    of its slowest cases once).  Returns (params, fields (n, L, Mn, 8), steps,
+    converged)."""
     params = sweep.sample_params(n, seed)
     fields = np.empty((n, L, M + 1, 8))
     steps = np.zeros(n, int)
