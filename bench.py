@@ -129,6 +129,9 @@ def main():
     ap.add_argument("--M", type=int, default=2560)
     ap.add_argument("--cpu-steps", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-order", type=int, default=None,
+                    help="stencil tile order (library default 0); for profiling the alternatives")
+    ap.add_argument("--tile-rows", type=int, default=None, help="stencil tile height (library default 8)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--warmstart-train", type=int, default=500)
     ap.add_argument("--warmstart-test", type=int, default=64)
@@ -147,6 +150,10 @@ def main():
     print(f"bench: {L}x{M} converging-diverging channel, {cells} cells, warmup {args.warmup}, "
           f"steps {args.steps}", file=sys.stderr, flush=True)
     with mpd_amd.Solver(workload_config(L, M)) as sol:
+        if args.tile_order is not None:
+            sol.set_option("tile_order", args.tile_order)
+        if args.tile_rows is not None:
+            sol.set_option("tile_rows", args.tile_rows)
         w0 = time.perf_counter()
         tm = sol.time_steps(args.steps, args.warmup)
         wall = time.perf_counter() - w0
@@ -178,6 +185,9 @@ def main():
         "step_breakdown_ms": {"stencil": round(tm["stencil_ms"], 5), "boundary": round(tm["bc_ms"], 5)},
         "wall_s": round(wall, 3),
     }
+    for k in ("tile_order", "tile_rows"):
+        if getattr(args, k) is not None:
+            line["config"][k] = getattr(args, k)
     if not args.no_sweep:
         print("bench: batched sweep", file=sys.stderr, flush=True)
         line["batched_sweep"] = sweep_measure()

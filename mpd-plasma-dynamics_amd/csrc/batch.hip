@@ -62,6 +62,9 @@ struct mpd_batch {
     // Work lists of the captured graph: stencil entries, boundary entries,
     // residual entries (in this order in d_work).
     std::vector<char> graph_active;
+    // Steps of the graph (0..K-1) that carry the convergence launch: those on
+    // which some active instance's step count reaches a check.
+    std::vector<char> graph_due;
     int n_st = 0, n_bc = 0, n_res = 0;
     int bc_words = 0;  // largest BC program image of the work lists' instances
     int bc_area = 0;   // largest boundary work area (slots per lane)
@@ -220,15 +223,37 @@ struct mpd_batch {
         HIPCHK(hipStreamSynchronize(stream));
     }
 
-    void EnqueueStep(int flip) {
+    void EnqueueStep(int flip, bool check) {
         HIPCHK(mpd::launch_stencil_batch(d_args, d_icur, d_work, n_st, grid, flip, fast_div, buf, 0.0, 1, stream));
         HIPCHK(mpd::launch_bc_batch(d_args, d_icur, d_work + n_st, n_bc, flip, bc_words, bc_area, stream));
         HIPCHK(mpd::launch_finish_batch(d_args, d_icur, n(), flip, stream));
-        if (n_res > 0) HIPCHK(mpd::launch_residual_batch(d_args, d_icur, d_aux, d_work + n_st + n_bc, n_res, flip,
-                                                         stream));
+        if (check && n_res > 0)
+            HIPCHK(mpd::launch_residual_batch(d_args, d_icur, d_aux, d_work + n_st + n_bc, n_res, flip, stream));
     }
 
-    void Capture(const std::vector<char>& active, int K) {
+    // Graph steps j in [0, K) on which an active instance checks: its step
+    // count after step j is step + j + 1, and the loop checks when that is a
+    // multiple of check_frequency (the residual kernel tests the same
+    // condition on the device step count, so a launch left out is one that
+    // would have returned at once).  An instance that stops inside the graph
+    // (done, or waiting for a rescan) keeps its step count: its check, if due,
+    // fell on the step where it stopped.  A rescan finished on the host moves
+    // the instance's phase, which changes this set and recaptures the graph.
+    std::vector<char> DueSteps(const std::vector<char>& active, int K) const {
+        std::vector<char> due(K, 0);
+        for (int i = 0; i < n(); ++i) {
+            if (!active[i]) continue;
+            const mpd_solver& s = *inst[i];
+            if (s.cfg.convergence_threshold <= 0.0) continue;
+            const int cf = s.cfg.check_frequency;
+            if (cf <= 0) continue;
+            int j = (cf - (s.step + 1) % cf) % cf;  // first j with (step + j + 1) % cf == 0
+            for (; j < K; j += cf) due[j] = 1;
+        }
+        return due;
+    }
+
+    void Capture(const std::vector<char>& active, const std::vector<char>& due, int K) {
         if (graph) {
             HIPCHK(hipGraphExecDestroy(graph));
             graph = nullptr;
@@ -238,12 +263,13 @@ struct mpd_batch {
         UploadWork(w);
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int j = 0; j < K; ++j) EnqueueStep(j & 1);
+        for (int j = 0; j < K; ++j) EnqueueStep(j & 1, due[j] != 0);
         HIPCHK(hipStreamEndCapture(stream, &gr));
         const hipError_t e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
         HIPCHK(e);
         graph_active = active;
+        graph_due = due;
         graph_K = K;
     }
 
@@ -302,7 +328,8 @@ struct mpd_batch {
             for (int i = 0; i < N; ++i) any = any || active[i];
             if (!any) break;
             const int K = ChooseK(active);
-            if (!graph || graph_K != K || graph_active != active) Capture(active, K);
+            const std::vector<char> due = DueSteps(active, K);
+            if (!graph || graph_K != K || graph_active != active || graph_due != due) Capture(active, due, K);
             for (int i = 0; i < N; ++i) h_icur.p[i] = inst[i]->cur;
             HIPCHK(hipMemcpyAsync(d_icur, h_icur.p, sizeof(int) * N, hipMemcpyHostToDevice, stream));
             HIPCHK(hipGraphLaunch(graph, stream));

@@ -181,9 +181,22 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
 
 // Record of row l for the marching kernel from its 8 conserved values; zero
 // physical values on the ghost rows and beyond the last interior column.
+// A per-row geometry value (row l wave-uniform) read through the constant
+// address space: the geometry arrays are not written during a step, and a
+// uniform load from that space is a scalar load.  Scalar loads have their
+// own completion counter; a vector load waited for in the march would also
+// wait for all of the wave's older stores (one counter for vector loads and
+// stores).
+__device__ __forceinline__ double row_value(const double* p, int l) {
+    using CDouble = const __attribute__((address_space(4))) double;
+    return ((CDouble*)p)[l];
+}
+
 __device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, const double* raw, Rec& c) {
     for (int k = 0; k < 8; ++k) c.u[k] = raw[k];
-    c.r = cell_r(a, l, m);
+    // cell_r (Grid::Build, src/grid.cpp:31-37) with scalar loads of rin, rout.
+    const double frac = (double)(m - 1) * a.dy;
+    c.r = (1.0 - frac) * row_value(a.rin, l) + frac * row_value(a.rout, l);
     rec_phys_shared(c, a.gm1);
     // Selects, not a branch around rec_phys_shared: two definitions of the
     // record merged at a join would put it in private memory.
@@ -314,7 +327,7 @@ __device__ __forceinline__ double speed_T(const StepArgs& a, int loop) {
 }
 
 struct TileCtx {
-    int m, l1;
+    int m, l0, l1;
     uint32_t st_off;
     bool outlane, store;
     double dt, dz2, dz2i;
@@ -322,18 +335,21 @@ struct TileCtx {
 };
 
 // One row of the march: output row l from the records S (l-1), C (l), N
-// (l+1); the loads of row l+2 are issued first and land in S, which the
-// caller rotates to the N slot.
-template <bool kSpeed, bool kBuf, class Div>
+// (l+1).  Marching down (rows l0 .. l1-1 in increasing order) the loads of
+// row l+2 are issued first and land in S, which the caller rotates to the N
+// slot; marching up (kUp: rows l1-1 .. l0) row l-2 lands in N.  The cell
+// arithmetic is the same either way.
+template <bool kSpeed, bool kBuf, class Div, bool kUp>
 __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, const StateIO<kBuf>& io, int l,
-                                          Rec& S, const Rec& C, const Rec& N, uint64_t& key) {
-    const bool pre = (l + 2 <= x.l1);
+                                          Rec& S, const Rec& C, Rec& N, uint64_t& key) {
+    const int lp = kUp ? l - 2 : l + 2;
+    const bool pre = kUp ? (lp >= x.l0 - 1) : (lp <= x.l1);
     double raw[8];
-    if (pre) io.load_row(l + 2, raw);
+    if (pre) io.load_row(lp, raw);
     const LaneNb nb{C};
-    const double dr2 = 2 * a.dr[l];
+    const double dr2 = 2 * row_value(a.dr, l);
     double o[8];
-    lf_cell_rad<Div>(N, S, C, nb, x.dt, x.dz2, x.dz2i, dr2, a.dr2inv[l], o);
+    lf_cell_rad<Div>(N, S, C, nb, x.dt, x.dz2, x.dz2i, dr2, row_value(a.dr2inv, l), o);
     if (l == 1 || l == a.L) {
         if (x.outlane) {
             char* rb = reinterpret_cast<char*>(a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch) + x.st_off;
@@ -349,7 +365,7 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
             }
         }
     }
-    if (pre) row_rec(a, l + 2, x.m, raw, S);
+    if (pre) row_rec(a, lp, x.m, raw, kUp ? N : S);
 }
 
 // The production stencil.  Each wavefront owns strip s (columns 62 s ..
@@ -365,16 +381,17 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
 // a row covers the whole slot: lane i writes slot (i + 63) mod 64, its own
 // cell for lanes 1..62 and an unused double for lanes 0 and 63; lanes of the
 // boundary and ghost columns 0, 1, M_max+1, M_max+2 do not store.
-// One wavefront's tile (strip, chunk of H rows) of the march; `tile` is
-// wave-uniform.
+// One wavefront's tile (strip, chunk of H rows) of the march; `tile` and
+// `up` (march the rows in decreasing order) are wave-uniform.
 template <bool kSpeed, bool kBuf, class Div>
 __device__ __forceinline__ void stencil_tile(const StepArgs& a, double dt_explicit, int loop, int H, int nstrips,
-                                             int tile) {
+                                             int tile, bool up = false) {
     const int lane = (int)(threadIdx.x & 63);
     const int strip = tile % nstrips, chunk = tile / nstrips;
     TileCtx x;
     x.m = kStripCols * strip - 1 + lane;
     const int l0 = 1 + chunk * H;
+    x.l0 = l0;
     x.l1 = min(l0 + H, a.L + 1);
     x.dt = step_dt(a, dt_explicit, loop);
     const bool central = x.m >= 2 && x.m <= a.Mn - 1;
@@ -389,24 +406,47 @@ __device__ __forceinline__ void stencil_tile(const StepArgs& a, double dt_explic
     // Lanes outside the columns 0 .. M_max+2 read the nearest ghost column.
     io.init(a, (uint32_t)phys_col(min(max(x.m, 0), a.Mn + 1)) * 8u, x.st_off);
     Rec A, B, C;
-    {
-        double raw[8];
-        io.load_row(l0 - 1, raw);
-        row_rec(a, l0 - 1, x.m, raw, A);
-        io.load_row(l0, raw);
-        row_rec(a, l0, x.m, raw, B);
-        io.load_row(l0 + 1, raw);
-        row_rec(a, l0 + 1, x.m, raw, C);
-    }
     uint64_t key = 0;
-    int l = l0;
-    for (; l + 3 <= x.l1; l += 3) {
-        march_row<kSpeed, kBuf, Div>(a, x, io, l, A, B, C, key);
-        march_row<kSpeed, kBuf, Div>(a, x, io, l + 1, B, C, A, key);
-        march_row<kSpeed, kBuf, Div>(a, x, io, l + 2, C, A, B, key);
+    if (!up) {
+        {
+            double raw[8];
+            io.load_row(l0 - 1, raw);
+            row_rec(a, l0 - 1, x.m, raw, A);
+            io.load_row(l0, raw);
+            row_rec(a, l0, x.m, raw, B);
+            io.load_row(l0 + 1, raw);
+            row_rec(a, l0 + 1, x.m, raw, C);
+        }
+        int l = l0;
+        for (; l + 3 <= x.l1; l += 3) {
+            march_row<kSpeed, kBuf, Div, false>(a, x, io, l, A, B, C, key);
+            march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 1, B, C, A, key);
+            march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 2, C, A, B, key);
+        }
+        if (l < x.l1) march_row<kSpeed, kBuf, Div, false>(a, x, io, l, A, B, C, key);
+        if (l + 1 < x.l1) march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 1, B, C, A, key);
+    } else {
+        // Rows l1-1 down to l0: A, B, C start as rows l1-2, l1-1, l1 (all
+        // within 0 .. L+1 since l0 >= 1 and l1 <= L+1).
+        const int l1 = x.l1;
+        {
+            double raw[8];
+            io.load_row(l1, raw);
+            row_rec(a, l1, x.m, raw, C);
+            io.load_row(l1 - 1, raw);
+            row_rec(a, l1 - 1, x.m, raw, B);
+            io.load_row(l1 - 2, raw);
+            row_rec(a, l1 - 2, x.m, raw, A);
+        }
+        int l = l1 - 1;
+        for (; l - 2 >= l0; l -= 3) {
+            march_row<kSpeed, kBuf, Div, true>(a, x, io, l, A, B, C, key);
+            march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 1, C, A, B, key);
+            march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 2, B, C, A, key);
+        }
+        if (l >= l0) march_row<kSpeed, kBuf, Div, true>(a, x, io, l, A, B, C, key);
+        if (l - 1 >= l0) march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 1, C, A, B, key);
     }
-    if (l < x.l1) march_row<kSpeed, kBuf, Div>(a, x, io, l, A, B, C, key);
-    if (l + 1 < x.l1) march_row<kSpeed, kBuf, Div>(a, x, io, l + 1, B, C, A, key);
     if (kSpeed) {
         key = wave_max_u64(key);
         if (lane == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
@@ -427,12 +467,29 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt
     // 4 g .. 4 g + 3 of it, with (q, g) counting the XCD's own workgroups:
     // each XCD sweeps whole chunks (all strips of a band of H rows), so the
     // edge columns two adjacent strips share are read into one L2, and the
-    // XCDs still advance together over a narrow band of rows.
+    // XCDs still advance together over a narrow band of rows.  Order 2: the
+    // four waves of a workgroup take four vertically adjacent tiles of one
+    // strip (chunks 4 G .. 4 G + 3), waves 0 and 2 marching down, 1 and 3 up,
+    // so each of the three rows two of them share (a tile's halo row is its
+    // neighbour's first or last row) is read by both waves at the same point
+    // of their marches, on one CU; the chunk groups go to the XCDs as the
+    // chunks of order 1 (workgroup w: group 8 q + (w mod 8), all strips of it
+    // one after another on that XCD).
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int tile;
+    bool up = false;
     if (order == 0) {
         tile = (int)blockIdx.x * 4 + wave;
         if (tile >= ntiles) return;
+    } else if (order == 2) {
+        const int nchunks = ntiles / nstrips;
+        const int w = (int)blockIdx.x;
+        const int j = w >> 3;
+        const int q = j / nstrips, strip = j - q * nstrips;
+        const int chunk = 4 * ((q << 3) + (w & 7)) + wave;
+        if (chunk >= nchunks) return;
+        tile = chunk * nstrips + strip;
+        up = (wave & 1) != 0;
     } else {
         const int gpc = (nstrips + 3) >> 2;  // workgroups per chunk
         const int w = (int)blockIdx.x;
@@ -443,7 +500,7 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt
         if (chunk >= ntiles / nstrips || strip >= nstrips) return;
         tile = chunk * nstrips + strip;
     }
-    stencil_tile<kSpeed, kBuf, Div>(a, dt_explicit, loop, H, nstrips, tile);
+    stencil_tile<kSpeed, kBuf, Div>(a, dt_explicit, loop, H, nstrips, tile, up);
 }
 
 // The tiled stencil over many instances of uneven size in one launch.  The
@@ -1316,8 +1373,10 @@ hipError_t launch_tiled(const StepArgs& a, int H, int order, double dt, int loop
     const int nstrips = (a.Mn - 1) / kStripCols + 1;
     const int nchunks = (a.L + H - 1) / H;
     const int ntiles = nstrips * nchunks;
-    const unsigned blocks = order == 0 ? (unsigned)((ntiles + 3) / 4)
-                                       : (unsigned)(8 * ((nchunks + 7) / 8) * ((nstrips + 3) / 4));
+    const int ngroups = (nchunks + 3) / 4;  // order 2: chunk groups of four
+    const unsigned blocks = order == 0   ? (unsigned)((ntiles + 3) / 4)
+                            : order == 1 ? (unsigned)(8 * ((nchunks + 7) / 8) * ((nstrips + 3) / 4))
+                                         : (unsigned)(8 * ((ngroups + 7) / 8) * nstrips);
     // Buffer addressing needs every row/field offset of a state below 2^31 bytes.
     const bool buf = (double)a.fstride * 8.0 * 8.0 < 2147483648.0;
     if (buf) {

@@ -232,7 +232,11 @@ int mpd_set_option(mpd_solver* s, const char* key, double value) {
         if (k == "kernel") s->scfg.kind = (int)value == 0 ? mpd::STENCIL_SIMPLE : mpd::STENCIL_TILED;
         else if (k == "fast_div") s->scfg.fast_div = value != 0.0;
         else if (k == "tile_rows") s->scfg.tile_rows = (int)value;
-        else if (k == "tile_order") s->scfg.order = (int)value != 0 ? 1 : 0;
+        else if (k == "tile_order") {
+            const int o = (int)value;
+            if (o < 0 || o > 2) throw ArgError("tile_order must be 0, 1 or 2");
+            s->scfg.order = o;
+        }
         else if (k == "steps_per_graph") s->steps_per_graph = (int)value;
         else throw ArgError("unknown option '" + k + "'");
         for (auto& g : s->graph)

@@ -21,8 +21,8 @@ def _compare(sol, o, label):
         assert same_bits(p[k], o.phys(k)), f"{label} {O.PHYS_NAMES[k]}: {describe_mismatch(p[k], o.phys(k))}"
 
 
-def _spike(L, M, steps):
-    text = cases.channel_config(L, M, max_steps=steps)
+def _spike(L, M, steps, **kw):
+    text = cases.channel_config(L, M, max_steps=steps, **kw)
     spike = '"0.1 + 40 * max(0, 1 - 300 * abs(z - 0.5)) * max(0, 1 - 250 * abs(r - 0.5))"'
     return text.replace("  v_z: 0.1\n", f"  v_z: {spike}\n")
 
@@ -176,3 +176,26 @@ def test_batch_instances_are_borrowed(mpd):
         res = b.run()
         assert [r["steps"] for r in res] == [5, 7]
         assert b.instance(1).download()[0][0].shape == (26, 13)
+
+
+def test_batch_checks_across_rescans(mpd):
+    """Graphs carry the convergence launch only on steps where some instance
+    checks.  An instance whose step waits for an exact wave-speed rescan
+    (finished on the host) changes phase against the graph; its later checks
+    (every 7 steps, threshold never met) and a converging instance's (every
+    10 steps) must still run on the oracle's steps: same SolutionChange after
+    the last check, same stop, same bits."""
+    texts = [_spike(60, 24, 50, threshold=1e-30, check_frequency=7),
+             _conv_config(16, 6, "1.0", "0.1", 2e-3, 10),
+             cases.channel_config(40, 16, max_steps=45, threshold=1e-30, check_frequency=9)]
+    with mpd.Batch(texts) as b:
+        res = b.run()
+        for i, t in enumerate(texts):
+            o, ores = run_oracle(t, 0)
+            print(f"instance {i}: {res[i]['steps']} steps, change {res[i]['change']:.6e}", flush=True)
+            assert res[i]["steps"] == ores["steps"], (i, res[i], ores)
+            assert bool(res[i]["converged"]) == ores["converged"], (i, res[i], ores)
+            assert res[i]["t"] == ores["t"] and res[i]["dt"] == ores["dt"], (i, res[i], ores)
+            # SolutionChange sums in a different order than the oracle.
+            assert abs(res[i]["change"] - ores["change"]) <= 1e-12 * abs(ores["change"]), (i, res[i], ores)
+            _compare(b.instance(i), o, f"instance {i}")

@@ -62,18 +62,20 @@ def test_tiled_odd_tile_rows(mpd):
             _compare_state(sol, o, f"tile_rows={h}")
 
 
-@pytest.mark.parametrize("rows", [1, 3, 8])
-def test_tiled_xcd_band_order(mpd, rows):
+@pytest.mark.parametrize("order,rows", [(1, 1), (1, 3), (1, 8), (2, 1), (2, 2), (2, 5), (2, 8), (2, 37)])
+def test_tiled_xcd_band_order(mpd, order, rows):
     """Tile order 1 (whole row bands per XCD, workgroups past the last chunk
-    or strip idle) gives the same bits, including chunk counts that are not a
-    multiple of 8 and strip counts that are not a multiple of 4."""
+    or strip idle) and order 2 (four vertically adjacent tiles per workgroup,
+    alternate waves marching up) give the same bits, including chunk counts
+    that are not a multiple of 4 or 8, strip counts that are not a multiple
+    of 4, and a single chunk (three of the four waves idle)."""
     text = cases.channel_config(37, 300, geometry="condiv", noise=1e-3, max_steps=9, tile_rows=rows)
-    print(f"tile_order=1 tile_rows={rows}", flush=True)
+    print(f"tile_order={order} tile_rows={rows}", flush=True)
     o, _ = run_oracle(text, 0)
     with mpd.Solver(text) as sol:
-        sol.set_option("tile_order", 1)
+        sol.set_option("tile_order", order)
         sol.run()
-        _compare_state(sol, o, f"tile_order=1 rows={rows}")
+        _compare_state(sol, o, f"tile_order={order} rows={rows}")
 
 
 def _spike_config(L=60, M=24, steps=80):
