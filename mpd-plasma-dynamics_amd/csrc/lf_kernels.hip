@@ -226,7 +226,7 @@ constexpr int kStoreNT = 2;
 
 template <bool kBuf>
 struct StateIO {
-    __amdgpu_buffer_rsrc_t rin, rout;
+    __amdgpu_buffer_rsrc_t rin, rout, rrow;
     const double* pin;
     double* pout;
     int64_t row_bytes, field_bytes;
@@ -241,7 +241,14 @@ struct StateIO {
         pout = a.u_out;
         if (kBuf) {
             rin = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(a.u_in), 0, -1, 0x00020000);
-            rout = __builtin_amdgcn_make_buffer_rsrc(a.u_out, 0, -1, 0x00020000);
+            // The store descriptors carry their buffer's exact size (8 fields;
+            // 16 rows for the row buffer): a lane that must not store gets
+            // the offset kNoStore, past the end, and the hardware drops its
+            // store (a raw buffer access at or beyond num_records is not
+            // performed).  Every lane then executes the stores, with no
+            // branch around them.
+            rout = __builtin_amdgcn_make_buffer_rsrc(a.u_out, 0, (int)(8 * a.fstride * 8), 0x00020000);
+            rrow = __builtin_amdgcn_make_buffer_rsrc(a.rowbuf, 0, (int)(16 * (int64_t)a.pitch * 8), 0x00020000);
         }
     }
     __device__ __forceinline__ void load_row(int l, double* raw) const {
@@ -255,6 +262,23 @@ struct StateIO {
             const char* row = reinterpret_cast<const char*>(pin) + l * row_bytes + ld_off;
             for (int k = 0; k < 8; ++k) raw[k] = *reinterpret_cast<const double*>(row + k * field_bytes);
         }
+    }
+    // Offset of a lane that does not store: at or beyond any buffer size the
+    // raw-buffer path allows (every buffer below 2^31 bytes).
+    static constexpr uint32_t kNoStore = 0x80000000u;
+    // Row l of the march's output from every lane, raw buffers only: rows 1
+    // and L (rb) go to the row buffer (field k of row 1 at its row k, of
+    // row L at row 8 + k), the others to the new state; lanes with `on`
+    // false store nothing.  Descriptor and offsets are wave-uniform selects,
+    // so every row issues the same eight stores with no branch around them.
+    __device__ __forceinline__ void store_out(int l, bool rb, const double* o, bool on) const {
+        const __amdgpu_buffer_rsrc_t r = rb ? rrow : rout;
+        const int base = rb ? (l == 1 ? 0 : 8) * (int)row_bytes : l * (int)row_bytes;
+        const int step = rb ? (int)row_bytes : (int)field_bytes;
+        const uint32_t vo = on ? st_off : kNoStore;
+        using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(rin, 0, 0, 0));
+        for (int k = 0; k < 8; ++k)
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, o[k]), r, vo, base + k * step, kStoreNT);
     }
     __device__ __forceinline__ void store_row(int l, const double* o) const {
         if (kBuf) {
@@ -350,13 +374,19 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
     const double dr2 = 2 * row_value(a.dr, l);
     double o[8];
     lf_cell_rad<Div>(N, S, C, nb, x.dt, x.dz2, x.dz2i, dr2, row_value(a.dr2inv, l), o);
-    if (l == 1 || l == a.L) {
-        if (x.outlane) {
-            char* rb = reinterpret_cast<char*>(a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch) + x.st_off;
-            for (int k = 0; k < 8; ++k) *reinterpret_cast<double*>(rb + (int64_t)k * a.pitch * 8) = o[k];
+    // With raw buffers every row issues the same eight stores, with no branch
+    // around them: one counter tracks the wave's vector loads and stores,
+    // and a path without them would make the compiler's wait for the
+    // prefetched row (issued before the stores) wait for the stores too.
+    const bool rb = (l == 1 || l == a.L);
+    if (kBuf) io.store_out(l, rb, o, rb ? x.outlane : x.store);
+    if (rb) {
+        if (!kBuf && x.outlane) {
+            char* rbp = reinterpret_cast<char*>(a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch) + x.st_off;
+            for (int k = 0; k < 8; ++k) *reinterpret_cast<double*>(rbp + (int64_t)k * a.pitch * 8) = o[k];
         }
     } else {
-        if (x.store) io.store_row(l, o);
+        if (!kBuf && x.store) io.store_row(l, o);
         if (kSpeed && x.outlane) {
             const float est = speed_estimate(o, C.r, x.gg1);
             if (!(est < x.thr)) {
