@@ -2,6 +2,7 @@
 #include "setup.h"
 
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 
 namespace mpd {
@@ -14,20 +15,27 @@ GeometryRegistry& GeometryRegistry::Instance() {
 }
 
 void GeometryRegistry::RegisterGeometry(std::string name, Factory factory) {
+    std::lock_guard<std::mutex> lock(mu_);
     factories_[std::move(name)] = std::move(factory);
 }
 
 std::unique_ptr<IGeometry> GeometryRegistry::Create(const std::string& type, const YNode& params) const {
-    auto it = factories_.find(type);
-    if (it == factories_.end()) {
-        throw std::runtime_error("GeometryRegistry: unknown geometry type '" + type +
-                                 "'.\n  Did you forget to call register_all_geometries(), or mistype the name in "
-                                 "config.yaml?");
+    Factory f;
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = factories_.find(type);
+        if (it == factories_.end()) {
+            throw std::runtime_error("GeometryRegistry: unknown geometry type '" + type +
+                                     "'.\n  Did you forget to call register_all_geometries(), or mistype the name in "
+                                     "config.yaml?");
+        }
+        f = it->second;
     }
-    return it->second(params);
+    return f(params);
 }
 
 std::vector<std::string> GeometryRegistry::Names() const {
+    std::lock_guard<std::mutex> lock(mu_);
     std::vector<std::string> n;
     for (const auto& kv : factories_) n.push_back(kv.first);
     return n;
@@ -121,12 +129,17 @@ private:
 }  // namespace
 
 void RegisterAllGeometries() {
-    GeometryRegistry& reg = GeometryRegistry::Instance();
-    reg.RegisterGeometry("straight_coaxial", [](const YNode& p) { return std::make_unique<StraightCoaxial>(p); });
-    reg.RegisterGeometry("converging_diverging",
-                         [](const YNode& p) { return std::make_unique<ConvergingDiverging>(p); });
-    reg.RegisterGeometry("short_cathode_cosine",
-                         [](const YNode& p) { return std::make_unique<ShortCathodeCosine>(p); });
+    // Once per process, whichever thread gets here first (a batch sets up
+    // its instances on several threads).
+    static std::once_flag once;
+    std::call_once(once, [] {
+        GeometryRegistry& reg = GeometryRegistry::Instance();
+        reg.RegisterGeometry("straight_coaxial", [](const YNode& p) { return std::make_unique<StraightCoaxial>(p); });
+        reg.RegisterGeometry("converging_diverging",
+                             [](const YNode& p) { return std::make_unique<ConvergingDiverging>(p); });
+        reg.RegisterGeometry("short_cathode_cosine",
+                             [](const YNode& p) { return std::make_unique<ShortCathodeCosine>(p); });
+    });
 }
 
 // ----------------------------------------------------------------- grid

@@ -6,6 +6,7 @@
 // include/ics/expression_ic.hpp:124-143).
 #pragma once
 
+#include <mutex>
 #include <functional>
 #include <memory>
 #include <string>
@@ -38,6 +39,8 @@ public:
 
 private:
     GeometryRegistry() = default;
+    // Batches build their instances' host setups on several threads at once.
+    mutable std::mutex mu_;
     std::unordered_map<std::string, Factory> factories_;
 };
 

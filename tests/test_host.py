@@ -218,3 +218,32 @@ def test_thruster_config_is_the_shipped_case(mpd):
         assert same_bits(u1[k], u2[k]), k
     for k in range(10):
         assert same_bits(p1[k], p2[k]), k
+
+
+def test_host_setup_from_many_threads(mpd):
+    """A batch builds its instances' host setups (config, geometry registry,
+    grid, IC, BC program) on several threads at once: the same config checked
+    from 8 threads together, many times over, gives every thread the same
+    answer (the geometry registry is shared process state)."""
+    import threading
+    texts = [cases.channel_config(20 + i, 8 + i % 5, geometry="condiv" if i % 2 else "straight", max_steps=5)
+             for i in range(16)]
+    expect = [mpd.config_check(t) for t in texts]
+    errors = []
+
+    def work(seed):
+        try:
+            for r in range(40):
+                i = (seed * 7 + r) % len(texts)
+                got = mpd.config_check(texts[i])
+                if got != expect[i]:
+                    errors.append((i, got, expect[i]))
+        except Exception as e:  # noqa: BLE001 -- reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=(s,)) for s in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors[:5]
