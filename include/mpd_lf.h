@@ -132,7 +132,9 @@ int mpd_write_vtk_fields(const char* yaml_text, const double* const* phys, const
 /* Kernel selection: key "kernel" (0 = simple, 1 = tiled), "fast_div" (0/1),
  * "tile_rows", "tile_order" (0: strip-fastest launch order, 1: whole row bands
  * per XCD, 2: four vertically adjacent tiles per workgroup marching in
- * alternate directions, chunk groups per XCD), "steps_per_graph". */
+ * alternate directions, chunk groups per XCD), "buffers" (1: raw buffer
+ * addressing for states below 2^31 bytes, the default; 0: the plain-pointer
+ * path larger states take), "steps_per_graph". */
 int mpd_set_option(mpd_solver* s, const char* key, double value);
 
 /* Benchmark helper: run `warmup` then `steps` loop steps with the current

@@ -1398,7 +1398,8 @@ hipError_t launch_simple(const StepArgs& a, double dt, int loop, int speed, hipS
 }
 
 template <class Div>
-hipError_t launch_tiled(const StepArgs& a, int H, int order, double dt, int loop, int speed, hipStream_t s) {
+hipError_t launch_tiled(const StepArgs& a, int H, int order, bool buffers, double dt, int loop, int speed,
+                        hipStream_t s) {
     // Strips holding central columns 2 .. M_max (strip s: columns 62 s .. 62 s + 61).
     const int nstrips = (a.Mn - 1) / kStripCols + 1;
     const int nchunks = (a.L + H - 1) / H;
@@ -1408,7 +1409,7 @@ hipError_t launch_tiled(const StepArgs& a, int H, int order, double dt, int loop
                             : order == 1 ? (unsigned)(8 * ((nchunks + 7) / 8) * ((nstrips + 3) / 4))
                                          : (unsigned)(8 * ((ngroups + 7) / 8) * nstrips);
     // Buffer addressing needs every row/field offset of a state below 2^31 bytes.
-    const bool buf = (double)a.fstride * 8.0 * 8.0 < 2147483648.0;
+    const bool buf = buffers && (double)a.fstride * 8.0 * 8.0 < 2147483648.0;
     if (buf) {
         if (speed) hipLaunchKernelGGL((lf_stencil_tiled<true, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles, order);
         else hipLaunchKernelGGL((lf_stencil_tiled<false, true, Div>), dim3(blocks), dim3(256), 0, s, a, dt, loop, H, nstrips, ntiles, order);
@@ -1434,8 +1435,8 @@ hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt, int
         return launch_simple<DivExact>(a, dt, loop, speed, s);
     }
     const int H = c.tile_rows > 0 ? c.tile_rows : auto_rows(a);
-    if (c.fast_div) return launch_tiled<DivConst>(a, H, c.order, dt, loop, speed, s);
-    return launch_tiled<DivExact>(a, H, c.order, dt, loop, speed, s);
+    if (c.fast_div) return launch_tiled<DivConst>(a, H, c.order, c.buffers, dt, loop, speed, s);
+    return launch_tiled<DivExact>(a, H, c.order, c.buffers, dt, loop, speed, s);
 }
 
 hipError_t launch_bc(const StepArgs& a, double dt, int loop, int speed, hipStream_t s) {

@@ -237,6 +237,7 @@ int mpd_set_option(mpd_solver* s, const char* key, double value) {
             if (o < 0 || o > 2) throw ArgError("tile_order must be 0, 1 or 2");
             s->scfg.order = o;
         }
+        else if (k == "buffers") s->scfg.buffers = value != 0.0;
         else if (k == "steps_per_graph") s->steps_per_graph = (int)value;
         else throw ArgError("unknown option '" + k + "'");
         for (auto& g : s->graph)

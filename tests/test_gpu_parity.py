@@ -78,6 +78,22 @@ def test_tiled_xcd_band_order(mpd, order, rows):
         _compare_state(sol, o, f"tile_order={order} rows={rows}")
 
 
+@pytest.mark.parametrize("order,rows", [(0, 8), (0, 3), (2, 5)])
+def test_tiled_pointer_path(mpd, order, rows):
+    """The tiled stencil's plain-pointer path (what states of 2^31 bytes or
+    more take; forced here with the "buffers" option): stores behind the lane
+    masks instead of out-of-range buffer offsets, rows 1 and L to the row
+    buffer through a pointer.  Same bits as the oracle."""
+    text = cases.channel_config(37, 300, geometry="condiv", noise=1e-3, max_steps=9, tile_rows=rows)
+    print(f"buffers=0 tile_order={order} tile_rows={rows}", flush=True)
+    o, _ = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        sol.set_option("buffers", 0)
+        sol.set_option("tile_order", order)
+        sol.run()
+        _compare_state(sol, o, f"buffers=0 order={order} rows={rows}")
+
+
 def _spike_config(L=60, M=24, steps=80):
     """Straight channel with a one-cell v_z spike at (z, r) = (0.5, 0.5): the
     LF averaging flattens it, so the max wave speed falls by far more than
