@@ -179,8 +179,6 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
     }
 }
 
-// Record of row l for the marching kernel from its 8 conserved values; zero
-// physical values on the ghost rows and beyond the last interior column.
 // A per-row geometry value (row l wave-uniform) read through the constant
 // address space: the geometry arrays are not written during a step, and a
 // uniform load from that space is a scalar load.  Scalar loads have their
@@ -192,6 +190,8 @@ __device__ __forceinline__ double row_value(const double* p, int l) {
     return ((CDouble*)p)[l];
 }
 
+// Record of row l for the marching kernel from its 8 conserved values; zero
+// physical values on the ghost rows and beyond the last interior column.
 __device__ __forceinline__ void row_rec(const StepArgs& a, int l, int m, const double* raw, Rec& c) {
     for (int k = 0; k < 8; ++k) c.u[k] = raw[k];
     // cell_r (Grid::Build, src/grid.cpp:31-37) with scalar loads of rin, rout.
