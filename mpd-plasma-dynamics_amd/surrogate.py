@@ -10,7 +10,7 @@ solver's grid.  Every forward and backward operation runs on the hand-written
 gfx950 kernels of libmpd_nn.so (include/mpd_nn.h), on device memory that
 library allocates; the host side is numpy.  The prediction becomes the initial state of the
 plain FP64 LF loop, which relaxes it to the convergence threshold: the solver
-is the judge, and a case whose residual stalls falls back to a cold start.
+decides, and a case whose residual stalls falls back to a cold start.
 
 No counterpart in the reference: its runs start from the expression or VTK
 initial condition (src/main.cpp:117-148).

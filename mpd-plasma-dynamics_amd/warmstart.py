@@ -1,6 +1,6 @@
 """The learned warm-start pipeline: steady states from the batched solver,
 training of the encoder-decoder on the GPU, prediction of initial states, and
-the warm-versus-cold comparison in which the plain LF loop is both the judge
+the warm-versus-cold comparison in which the plain LF loop is both the arbiter
 (each warm start is relaxed to the same convergence threshold as a cold
 start) and the fallback (a case whose residual stalls starts again cold).
 

@@ -11,7 +11,7 @@ on a small grid (48 x 23: the 4-level network runs on 48 x 24 cells).
 * Junk predictions (negative density, a NaN) are not used: those cases
   start cold, and reproduce the cold steady state bit for bit; an
   admissible but poor prediction is still relaxed to the same steady state
-  (the solver is the judge).
+  (the solver decides).
 """
 import numpy as np
 import pytest
