@@ -168,10 +168,10 @@ struct mpd_batch {
         return (int)std::max(1LL, std::min(256LL, (cells + 4095) / 4096));
     }
 
-    static int RingBlocks(const mpd_solver& s) {
-        const int nring = 2 * s.grid.Mn + 2 * (s.grid.L - 2);
-        return (nring + 63) / 64;
-    }
+    // The batched boundary kernel uses the compact ring layout: its
+    // workgroups (LDS-bound residency) then all fit at once for many small
+    // instances.
+    static int RingBlocks(const mpd_solver& s) { return (mpd::ring_lanes_compact(s.grid.L, s.grid.Mn) + 63) / 64; }
 
     // Work lists over the instances flagged in `active`.  Larger instances
     // first, so the longest chains of tiles start earliest.

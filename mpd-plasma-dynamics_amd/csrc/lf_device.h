@@ -111,6 +111,50 @@ inline void build_bc_image(const MpdBcProgram& p, uint64_t* out) {
 // expression symbol table, the chain state and its neighbour state, then the
 // evaluation stack, as deep as the program's deepest expression needs.
 enum { WA_SYM = 0, WA_CUR = WA_SYM + MPD_BS_COUNT, WA_NBR = WA_CUR + 10, WA_STK = WA_NBR + 10 };
+// Boundary-kernel lanes of the ring of an L x Mn grid.  Compact (the
+// batched kernel, where many instances' workgroups must fit at once): row 1,
+// row L, column 1 (l = 2..L-1), column Mn, back to back.  Grouped (a single
+// grid): 64-lane groups of one kind of cell each: the interior cells of row 1 (m = 2..Mn-1), of row L,
+// of column 1 (l = 2..L-1) and of column Mn, each padded to whole groups,
+// then one group per corner (1,1), (1,Mn), (L,1), (L,Mn) with its first lane
+// only.  A 64-lane wave then evaluates one face's conditions, and a corner
+// (whose evaluation may also run an earlier face's chain at its neighbour)
+// runs alone: lanes of different kinds in one wave would take their paths
+// one after another.
+__host__ __device__ inline int ring_lanes_compact(int L, int Mn) { return 2 * Mn + 2 * (L - 2); }
+
+__host__ __device__ inline bool ring_cell_compact(int idx, int L, int Mn, int& l, int& m) {
+    if (idx < Mn) { l = 1; m = idx + 1; }
+    else if (idx < 2 * Mn) { l = L; m = idx - Mn + 1; }
+    else if (idx < 2 * Mn + L - 2) { l = idx - 2 * Mn + 2; m = 1; }
+    else { l = idx - (2 * Mn + L - 2) + 2; m = Mn; }
+    return idx < ring_lanes_compact(L, Mn);
+}
+
+__host__ __device__ inline int ring_lanes(int L, int Mn) {
+    const int g1 = ((Mn - 2) + 63) / 64 * 64, gc = ((L - 2) + 63) / 64 * 64;
+    return 2 * g1 + 2 * gc + 4 * 64;
+}
+
+// Cell (l, m) of ring lane idx; false for a padding lane.
+__host__ __device__ inline bool ring_cell(int idx, int L, int Mn, int& l, int& m) {
+    const int n1 = Mn - 2, nc = L - 2;
+    const int g1 = (n1 + 63) / 64 * 64, gc = (nc + 63) / 64 * 64;
+    if (idx < g1) { l = 1; m = 2 + idx; return idx < n1; }
+    idx -= g1;
+    if (idx < g1) { l = L; m = 2 + idx; return idx < n1; }
+    idx -= g1;
+    if (idx < gc) { l = 2 + idx; m = 1; return idx < nc; }
+    idx -= gc;
+    if (idx < gc) { l = 2 + idx; m = Mn; return idx < nc; }
+    idx -= gc;
+    const int c = idx / 64;
+    if (idx % 64 != 0 || c >= 4) return false;
+    l = (c < 2) ? 1 : L;
+    m = (c % 2 == 0) ? 1 : Mn;
+    return true;
+}
+
 __host__ __device__ inline int bc_area_slots(const MpdBcProgram& p) {
     return WA_STK + (p.max_stack > 1 ? p.max_stack : 1);
 }

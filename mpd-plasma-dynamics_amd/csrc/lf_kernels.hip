@@ -567,6 +567,7 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled_batch(const StepArgs*
 // Laid out [slot][lane] for the 64-thread workgroups of lf_bc_ring, so one
 // slot of a wave is 64 consecutive doubles.
 #define BC_THREADS 64
+static_assert(BC_THREADS == 64, "ring_lanes groups one kind of cell per 64-lane boundary workgroup");
 struct WorkArea {
     double* base;  // LDS: bc_area_slots * BC_THREADS doubles (slot layout WA_* in lf_device.h)
     int lane;
@@ -969,20 +970,16 @@ __device__ __forceinline__ void stage_image(const uint64_t* src, uint64_t* prog,
 // loop step (ticket and device-scope fences); the batched path finishes its
 // instances' steps in a kernel of its own instead (lf_finish_batch).
 __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt, int loop, int speed, int bid,
-                                              int nblk, const uint64_t* prog, double* area, bool finish) {
+                                              int nblk, const uint64_t* prog, double* area, bool finish,
+                                              bool grouped) {
     StepArgs a = a_in;
     a.bc = prog;  // the staged image, read through img(a)
     const WorkArea w{area, (int)threadIdx.x};
     const int L = a.L, Mn = a.Mn;
-    const int nring = 2 * Mn + 2 * (L - 2);
     const int idx = bid * BC_THREADS + (int)threadIdx.x;
     uint64_t key = 0;
-    if (idx < nring) {
-        int l, m;
-        if (idx < Mn) { l = 1; m = idx + 1; }
-        else if (idx < 2 * Mn) { l = L; m = idx - Mn + 1; }
-        else if (idx < 2 * Mn + L - 2) { l = idx - 2 * Mn + 2; m = 1; }
-        else { l = idx - (2 * Mn + L - 2) + 2; m = Mn; }
+    int l = 0, m = 0;
+    if (grouped ? ring_cell(idx, L, Mn, l, m) : ring_cell_compact(idx, L, Mn, l, m)) {
         // Last face covering the cell (M_LO skips l = 1 and l = L by its corner
         // policy, so it never owns a row-1 or row-L cell).
         int f = -1;
@@ -1058,7 +1055,7 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
     stage_image(a_in.bc, bc_lds, a_in.bc_words);
     if (idle) return;
     bc_ring_block(a_in, dt, loop, speed, (int)blockIdx.x, (int)gridDim.x, bc_lds,
-                  reinterpret_cast<double*>(bc_lds + a_in.bc_words), true);
+                  reinterpret_cast<double*>(bc_lds + a_in.bc_words), true, true);
 }
 
 // The boundary kernel over many instances: work entry b = (instance, block
@@ -1077,7 +1074,8 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring_batch(const StepArgs* _
     const double dt = step_dt(a, 0.0, 1);
     stage_image(a.bc, bc_lds, a.bc_words);
     if (idle) return;
-    bc_ring_block(a, dt, 1, speed, w.first, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words), false);
+    bc_ring_block(a, dt, 1, speed, w.first, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words), false,
+                  false);
 }
 
 // The step's end for every instance of a batch that took it (after the
@@ -1440,7 +1438,7 @@ hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt, int
 }
 
 hipError_t launch_bc(const StepArgs& a, double dt, int loop, int speed, hipStream_t s) {
-    const int nring = 2 * a.Mn + 2 * (a.L - 2);
+    const int nring = ring_lanes(a.L, a.Mn);
     hipLaunchKernelGGL(lf_bc_ring, dim3((unsigned)((nring + BC_THREADS - 1) / BC_THREADS)), dim3(BC_THREADS),
                        bc_lds_bytes(a.bc_words, a.bc_area), s, a, dt, loop, speed);
     return hipGetLastError();
