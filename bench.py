@@ -109,7 +109,8 @@ def warmstart_measure(train, test, epochs, L=96, M=39):
     losses = WS.train(net, SG.features(params, radii, L, M), norm.encode(fields - base).astype(np.float32), epochs,
                       32, 2e-3, log_every=max(1, epochs // 4))
     train_s = time.perf_counter() - t1
-    r, _, _ = WS.evaluate(net, norm, sweep.sample_params(test, 2025), L, M, plain_sample=8)
+    r, _, _ = WS.evaluate(net, norm, sweep.sample_params(test, 2025), L, M, plain_sample=8,
+                          budget=WS.step_budget(steps[conv]))
     keep = ["plain_cold_total_s", "cold_batch_s", "warm_total_s", "speedup_vs_plain_cold", "speedup_vs_cold_batch",
             "steps_ratio_cold_over_warm", "fallbacks", "all_converged_warm", "warm_finish_p50_s",
             "warm_finish_p99_s", "max_field_err_warm_vs_cold"]

@@ -243,13 +243,23 @@ def stalled(done, converged, change, change_mid, stall_change):
     return bool(change > stall_change and not (change < change_mid))
 
 
+def step_budget(train_steps, factor=1.5):
+    """Most steps a warm-started case may take before it is restarted cold:
+    factor times the slowest cold solve among the training states (a warm
+    start that needs more than any cold start did has gone wrong)."""
+    train_steps = np.asarray(train_steps)
+    return int(factor * train_steps.max()) if train_steps.size else None
+
+
 def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_change=1e-6, plain_sample=None,
-             chunk=1000, fallback=True, pred=None):
+             chunk=1000, fallback=True, pred=None, budget=None):
     """Cold-started plain solves (each case alone, mpd_run) against the
     surrogate-warm-started batched sweep, both to the same threshold.  A
     warm case whose SolutionChange is still above stall_change after
-    stall_steps steps is restarted cold (fallback).  plain_sample: time only
-    that many plain solves and scale to all cases (stated in the result)."""
+    stall_steps steps is restarted cold (fallback), and so is one still
+    running after `budget` steps (step_budget; None: no budget).
+    plain_sample: time only that many plain solves and scale to all cases
+    (stated in the result)."""
     n = len(params)
     cfgs = [sweep.case_config(p, L, M, threshold=threshold, max_steps=2000000) for p in params]
     radii = grid_radii(params, L, M)
@@ -303,6 +313,8 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     for i in stall:
         b.instance(i).cold_restart(cfgs[i])
         finish[i] = np.nan
+    restarted = set(stall) | set(rejected)
+    late = []
     while True:
         res = b.run(chunk)
         el = time.perf_counter() - t1
@@ -310,13 +322,20 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
         for i, r in enumerate(res):
             if r["done"] and np.isnan(finish[i]):
                 finish[i] = el
+        # Step budget: a warm case still running past it starts again cold.
+        if fallback and budget is not None:
+            for i, r in enumerate(res):
+                if not r["done"] and i not in restarted and steps[i] > budget:
+                    b.instance(i).cold_restart(cfgs[i])
+                    restarted.add(i)
+                    late.append(i)
         if all(r["done"] for r in res):
             break
     warm_wall = time.perf_counter() - t1
     warm_fields = steady_fields(b, n, L, M)
     b.close()
     progress(f"warm batch: inference {infer:.3f}s, relax {warm_wall:.2f}s, {len(rejected)} rejected, "
-             f"{len(stall)} stalled")
+             f"{len(stall)} stalled, {len(late)} over the step budget")
 
     errs = np.array([field_errors(warm_fields[i], cold_fields[i]) for i in range(n)])
     pred_errs = np.array([field_errors(pred[i], cold_fields[i]) for i in range(n)])
@@ -331,8 +350,8 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
         "warm_steps_p50": float(np.median(steps)), "warm_steps_max": int(steps.max()),
         "steps_saved_mean": float(np.mean(cold_steps - steps)),
         "steps_ratio_cold_over_warm": float(np.sum(cold_steps) / np.sum(steps)),
-        "fallbacks": len(rejected) + len(stall), "rejected_predictions": len(rejected),
-        "stalled": len(stall),
+        "fallbacks": len(rejected) + len(stall) + len(late), "rejected_predictions": len(rejected),
+        "stalled": len(stall), "over_budget": len(late), "step_budget": budget,
         "all_converged_cold": bool(all(r["converged"] for r in cold_res)),
         "all_converged_warm": bool(all(r["converged"] for r in res)),
         "speedup_vs_plain_cold": plain_total / (infer + warm_wall),
@@ -380,7 +399,8 @@ def main():
     losses = train(net, X, T, args.epochs, args.batch, args.lr, seed=args.seed, clip=args.clip, stats=tstats)
     train_s = time.perf_counter() - t1
     test_params = sweep.sample_params(args.test, args.seed + 1)
-    result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=args.plain_sample)
+    result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=args.plain_sample,
+                            budget=step_budget(steps[keep]))
     result.update({"train_cases": int(keep.sum()), "generate_s": gen_s, "train_s": train_s, "epochs": args.epochs,
                    "clip": args.clip, "rollbacks": tstats["rollbacks"],
                    "final_loss": losses[-1], "first_loss": losses[0], "n_params": net.n_params()})

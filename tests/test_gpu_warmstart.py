@@ -97,3 +97,19 @@ def test_junk_warm_start_falls_back_to_cold(ws, trained):
     # The fallback is the cold run itself.
     assert np.array_equal(warm[:2], cold[:2])
     assert max(ws.field_errors(warm[i], cold[i])[k] for i in (2, 3) for k in range(8)) < 1e-5
+
+
+def test_step_budget_restarts_cold(ws, trained):
+    """A warm case still running past the step budget starts again cold (a
+    prediction that relaxes too slowly costs at most the budget): with a
+    budget below the stall check's steps every case still running after the
+    check is restarted, and ends on the cold steady state bit for bit."""
+    import sweep
+    net, norm = trained
+    params = sweep.sample_params(3, 79)
+    r, warm, cold = ws.evaluate(net, norm, params, L, M, plain_sample=1, budget=1)
+    print({k: r[k] for k in ("over_budget", "stalled", "rejected_predictions", "fallbacks")}, flush=True)
+    assert r["all_converged_warm"] and r["all_converged_cold"]
+    assert r["over_budget"] >= 1 and r["step_budget"] == 1
+    assert r["fallbacks"] == r["over_budget"] + r["stalled"] + r["rejected_predictions"]
+    assert sum(np.array_equal(warm[i], cold[i]) for i in range(3)) >= r["over_budget"]

@@ -20,6 +20,14 @@ def test_stall_rule():
     assert not WS.stalled(False, False, 5e-7, 1e-7, 1e-6)
 
 
+def test_step_budget():
+    """The warm-start step budget: 1.5 x the slowest training case's cold
+    step count (none without training cases)."""
+    assert WS.step_budget([1000, 4000, 2500]) == 6000
+    assert WS.step_budget(np.array([7]), factor=2.0) == 14
+    assert WS.step_budget([]) is None
+
+
 def test_admissible():
     f = np.ones((4, 3, 8))
     assert WS.admissible(f)
