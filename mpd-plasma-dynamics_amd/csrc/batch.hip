@@ -161,12 +161,9 @@ struct mpd_batch {
         HIPCHK(hipMemcpy(d_aux, x.data(), x.size() * sizeof(mpd::BatchAux), hipMemcpyHostToDevice));
     }
 
-    // Residual workgroups of one instance (the per-instance partial buffer
-    // holds 1024 block sums).
-    static int ResBlocks(const mpd_solver& s) {
-        const long long cells = (long long)s.grid.L * s.grid.Mn;
-        return (int)std::max(1LL, std::min(256LL, (cells + 4095) / 4096));
-    }
+    // Residual workgroups of one instance: the single solver's partition
+    // (the per-instance partial buffer holds its up to 1024 block sums).
+    static int ResBlocks(const mpd_solver& s) { return mpd::residual_blocks(s.grid.L, s.grid.Mn); }
 
     // The batched boundary kernel uses the compact ring layout: its
     // workgroups (LDS-bound residency) then all fit at once for many small

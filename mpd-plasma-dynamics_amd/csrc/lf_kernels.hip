@@ -1480,12 +1480,6 @@ hipError_t launch_residual_batch(const StepArgs* args, const int* icur, const Ba
     return hipGetLastError();
 }
 
-int residual_blocks(const StepArgs& a) {
-    const long long n = (long long)a.L * a.Mn;
-    long long b = (n + 255) / 256;
-    return (int)(b < 1024 ? b : 1024);
-}
-
 hipError_t launch_speed_fallback(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(lf_speed_fallback, dim3(256), dim3(256), 0, s, a);
     return hipGetLastError();
@@ -1493,7 +1487,7 @@ hipError_t launch_speed_fallback(const StepArgs& a, hipStream_t s) {
 
 hipError_t launch_residual(const StepArgs& a, double* prev, double* partial, int loop, int check_frequency,
                            double threshold, int save, hipStream_t s) {
-    hipLaunchKernelGGL(lf_residual, dim3((unsigned)residual_blocks(a)), dim3(256), 0, s, a, prev, partial, loop,
+    hipLaunchKernelGGL(lf_residual, dim3((unsigned)residual_blocks(a.L, a.Mn)), dim3(256), 0, s, a, prev, partial, loop,
                        check_frequency, threshold, save);
     return hipGetLastError();
 }

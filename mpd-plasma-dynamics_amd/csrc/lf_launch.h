@@ -45,7 +45,13 @@ hipError_t launch_speed_fallback(const StepArgs& a, hipStream_t s);
 // SolutionChange (+ SavePrev when save).  loop: run only on a check step.
 hipError_t launch_residual(const StepArgs& a, double* prev, double* partial, int loop, int check_frequency,
                            double threshold, int save, hipStream_t s);
-int residual_blocks(const StepArgs& a);
+// Workgroups of one instance's SolutionChange reduction: every launch of it
+// (single solver, batch, rescan path) partitions the cells the same way, so
+// an instance's change has the same bits however it is run.
+inline int residual_blocks(int L, int Mn) {
+    const long long b = ((long long)L * Mn + 255) / 256;
+    return (int)(b < 1 ? 1 : (b < 1024 ? b : 1024));
+}
 
 // Batched launches over many instances (work lists built by the host; args
 // holds two StepArgs per instance, icur the start buffer of each instance).

@@ -196,14 +196,33 @@ def unet_layers(cin, cout, base=32, levels=4):
     return layers
 
 
+def padded(n, levels=4):
+    """n rounded up to a multiple of 2^(levels-1): the size the network's
+    pooling levels need."""
+    q = 2 ** (levels - 1)
+    return (n + q - 1) // q * q
+
+
+def pad_grid(a, H, W):
+    """(N, h, w, C) array edge-replicated to (N, H, W, C) (H >= h, W >= w)."""
+    a = np.asarray(a)
+    h, w = a.shape[1], a.shape[2]
+    if (h, w) == (H, W):
+        return a
+    return np.pad(a, ((0, 0), (0, H - h), (0, W - w), (0, 0)), mode="edge")
+
+
 class UNet:
     """The encoder-decoder on the GPU: FP32 master weights with bf16 copies,
     Adam moments, and the activations of the last forward pass (kept for the
-    backward pass)."""
+    backward pass).  A grid of h x w cells runs on the network's padded size
+    H x W (each rounded up to a multiple of 2^(levels-1)): inputs and
+    training targets are edge-replicated to it (pad_grid) and outputs cropped
+    back to h x w."""
 
-    def __init__(self, H, W, cin=N_FEATURES, cout=len(OUT_FIELDS), base=32, levels=4, seed=0):
-        if H % 2 ** (levels - 1) or W % 2 ** (levels - 1):
-            raise ValueError(f"grid {H}x{W} must be divisible by {2 ** (levels - 1)}")
+    def __init__(self, h, w, cin=N_FEATURES, cout=len(OUT_FIELDS), base=32, levels=4, seed=0):
+        self.h, self.w = h, w
+        H, W = padded(h, levels), padded(w, levels)
         self.H, self.W, self.cin, self.cout, self.base, self.levels = H, W, cin, cout, base, levels
         self.layers = unet_layers(cin, cout, base, levels)
         rng = np.random.default_rng(seed)

@@ -135,6 +135,8 @@ def train(net, X, T, epochs, batch, lr, seed=0, log_every=20, clip=None, guard=3
     stuck at a constant output).  clip: optional global gradient-norm clip.
     Returns the per-epoch mean losses; `stats` (a dict) receives the number
     of clipped steps, the largest gradient norm and the rollbacks."""
+    X = SG.pad_grid(X, net.H, net.W)
+    T = SG.pad_grid(T, net.H, net.W)
     N, H, W, C = X.shape
     Xd = SG.DeviceArray.from_numpy(X.reshape(N, -1))
     Td = SG.DeviceArray.from_numpy(T.reshape(N, -1))
@@ -195,15 +197,16 @@ def predict(net, norm, params, radii, L, M, batch=32):
     n = len(params)
     X = SG.features(params, radii, L, M)
     base = SG.baseline_fields(params, radii, L, M, GAMMA)
+    X = SG.pad_grid(X, net.H, net.W)
     out = np.empty((n, L, M + 1, 8))
-    xb16 = SG.DeviceArray((batch, L, M + 1, SG.N_FEATURES), "bf16")
+    xb16 = SG.DeviceArray((batch, net.H, net.W, SG.N_FEATURES), "bf16")
     for s0 in range(0, n, batch):
         k = min(batch, n - s0)
         xx = np.zeros((batch,) + X.shape[1:], np.float32)
         xx[:k] = X[s0:s0 + k]
         xb16.upload(xx)
-        y = net.forward(xb16, batch).download()
-        out[s0:s0 + k] = base[s0:s0 + k] + norm.decode(y[:k].astype(np.float64))
+        y = net.forward(xb16, batch).download()[:k, :L, :M + 1]
+        out[s0:s0 + k] = base[s0:s0 + k] + norm.decode(y.astype(np.float64))
     return out
 
 

@@ -435,6 +435,18 @@ static double axis_u7(const orc_solver* s, int l, int m, double dt) {
            AT(r, l, m);
 }
 
+/* Test hook: AxisLfU1/U2/U5/U7 (k = 0, 1, 4, 6: the conserved index it
+ * overwrites) at cell (l, m) on the current arrays; NaN for another k. */
+double orc_axis_lf(const orc_solver* s, int k, int l, int m, double dt) {
+    switch (k) {
+        case 0: return axis_u1(s, l, m, dt);
+        case 1: return axis_u2(s, l, m, dt);
+        case 4: return axis_u5(s, l, m, dt);
+        case 6: return axis_u7(s, l, m, dt);
+        default: return NAN;
+    }
+}
+
 /* The 8 physical arrays in the BC evaluation order rho, v_z, v_r, v_phi,
  * H_z, H_r, H_phi, e (src/bcs/per_field_bc.cpp:322-360). */
 static const int kBcField[8] = {ORC_RHO, ORC_VZ, ORC_VR, ORC_VPHI, ORC_HZ, ORC_HR, ORC_HPHI, ORC_E};

@@ -98,6 +98,10 @@ double orc_mass_flux(const orc_solver* s);
 double orc_thrust(const orc_solver* s);
 double orc_radial_symmetry(const orc_solver* s);
 
+/* Test hook: PerFieldBC::AxisLfU1/U2/U5/U7 (k = 0, 1, 4, 6) at (l, m) on the
+ * current u0, physical and grid arrays (src/bcs/per_field_bc.cpp:490-566). */
+double orc_axis_lf(const orc_solver* s, int k, int l, int m, double dt);
+
 /* Seeded uniform in [0,1) of the density noise: SplitMix64 of (seed, idx). */
 double orc_noise_u01(unsigned long long seed, unsigned long long idx);
 

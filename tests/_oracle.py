@@ -81,6 +81,8 @@ def lib():
         L.orc_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.orc_integrals.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.orc_axis_lf.restype = C.c_double
+        L.orc_axis_lf.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double]
         L.orc_noise_u01.restype = C.c_double
         L.orc_noise_u01.argtypes = [C.c_ulonglong, C.c_ulonglong]
         _lib = L
@@ -239,6 +241,10 @@ class Oracle:
         ch = C.c_double(0)
         n = lib().orc_run(self.h, max_steps, C.byref(tt), C.byref(dd), C.byref(conv), C.byref(ch))
         return dict(steps=n, t=tt.value, dt=dd.value, converged=bool(conv.value), change=ch.value)
+
+    def axis_lf(self, k, l, m, dt):
+        """AxisLfU(k+1) at (l, m) on the current arrays (k = 0, 1, 4, 6)."""
+        return lib().orc_axis_lf(self.h, k, l, m, dt)
 
     def integrals(self):
         out = (C.c_double * 7)()

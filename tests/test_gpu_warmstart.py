@@ -1,5 +1,6 @@
 """GPU tests of the learned warm start (mpd-plasma-dynamics_amd/warmstart.py)
-on a small grid (48 x 23: the 4-level network runs on 48 x 24 cells).
+on a small grid: 48 x 24 (48 x 25 cells; the 4-level network pads them to
+48 x 32 and crops its output back).
 
 * The cold steady state the sweeps are built on is the oracle's, bit for bit
   (the step count at which the residual falls below 1e-8 included).
@@ -22,7 +23,7 @@ from parity import describe_mismatch, same_bits
 
 pytestmark = pytest.mark.gpu
 
-L, M = 48, 23
+L, M = 48, 24
 
 
 @pytest.fixture(scope="module")

@@ -97,3 +97,17 @@ def test_case_config_parses():
     p = sweep.sample_params(2, 9)
     for q in p:
         assert mpd_amd.config_check(sweep.case_config(q, 20, 8)) == (20, 8)
+
+
+def test_padding_for_the_network_levels():
+    """Grids whose sides are not multiples of 8 run on the padded size:
+    edge replication up, cropping back is exact."""
+    assert SG.padded(25) == 32 and SG.padded(24) == 24 and SG.padded(161) == 168 and SG.padded(321) == 328
+    a = np.arange(2 * 5 * 3 * 2, dtype=np.float64).reshape(2, 5, 3, 2)
+    p = SG.pad_grid(a, 8, 8)
+    assert p.shape == (2, 8, 8, 2)
+    assert np.array_equal(p[:, :5, :3], a)
+    assert np.array_equal(p[:, 5:, :3], np.repeat(a[:, 4:5], 3, axis=1))
+    assert np.array_equal(p[:, :5, 3:], np.repeat(a[:, :, 2:3], 5, axis=2))
+    assert np.all(p[:, 5:, 3:] == a[:, 4:5, 2:3])
+    assert SG.pad_grid(a, 5, 3) is a
