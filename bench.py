@@ -1,27 +1,42 @@
 """Benchmark of the FP64 Lax-Friedrichs step on one MI355X.
 
-Also on the line: `batched_sweep` (64 uneven instances at the sweep sizes, batched
-against one at a time) and `steady_state` (a reduced live run of the learned warm
-start: wall time to residual 1e-8 of surrogate-warm-started batched solves against
-cold-started plain solves; --no-warmstart skips it).
-
-Workload: the memory-bound scaling case, the converging-diverging channel
-r_in = 0.2 + 0.05 sin(pi z), r_out = 0.8 - 0.1 z^2 at 6400 x 2560 (L_max x
-M_max; 16,390,400 interior cells), IC rho = 1 (+1e-3 seeded noise), u_z = 0.1,
-B_phi = 0.2 / r, p = 0.5, inflow / outflow / slip walls, adaptive CFL 0.4.
-A step is one full time step of the loop (stencil, boundary faces, CFL dt
-update), with the state resident in HBM.
+Default mode (the one JSON line): the memory-bound scaling case, the
+converging-diverging channel r_in = 0.2 + 0.05 sin(pi z), r_out = 0.8 - 0.1 z^2
+at 6400 x 2560 (L_max x M_max; 16,390,400 interior cells), IC rho = 1 (+1e-3
+seeded noise), u_z = 0.1, B_phi = 0.2 / r, p = 0.5, inflow / outflow / slip
+walls, adaptive CFL 0.4.  A step is one full time step of the loop (stencil,
+boundary faces, CFL dt update), with the state resident in HBM.
 
   value  = GCUPS = interior cells x steps / seconds / 1e9
-  roofline.achieved = 128 B (8 FP64 conserved values read + 8 written) x cells
-                      / the stencil kernel's average launch time (HIP events)
+  roofline.achieved = 128 B (8 FP64 conserved values read + 8 written)
+                      x cells x steps / seconds of the whole timed loop
+  kernel_roofline   = the same bytes / the stencil kernel's average launch
+                      time (HIP events on the solver's stream)
 
-The CPU baseline times the oracle (the C restatement of the reference's
-six-pass step) on a bounded sample of the same workload on this host.
+Also on the line: `batched_sweep` (64 uneven instances at the sweep sizes,
+batched against one at a time), `steady_state` (a reduced live run of the
+learned warm start; --no-warmstart skips it), the measured copy ceiling of
+the HBM, and `cpu_baseline`: the reference's own compiled step
+(oracle/_ref/ref_harness, built from the reference sources in the build
+container; it travels with the tree) on a bounded sample of the same
+workload on this host's physical cores, in a child process pinned as the
+reference's job script pins its OpenMP threads (OMP_PROC_BIND=close,
+OMP_PLACES=cores), with a one-thread figure beside it, and the oracle (the C
+restatement of the reference's six-pass step) timed the same way as `port`.
+
+Other modes (each prints its own report; the profiles' commands use them):
+  --mode copy            the copy kernel: counter calibration + copy ceiling
+  --mode sweep-probe     a batched sweep workload timed (--probe small|uneven)
+  --mode sweep-converge  uneven sweep cases to residual 1e-8, batched vs one at a time
+  --mode warmstart       the full learned warm-start experiment
+  --mode nn-train        training steps of the warm-start network
+  --mode cpu-calibration the oracle against the reference-compiled harness
+                         (build container only: needs oracle/_ref)
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -30,6 +45,7 @@ sys.path[:0] = [os.path.join(REPO, "mpd-plasma-dynamics_amd"), os.path.join(REPO
 
 HBM_PEAK_GBS = 8000.0
 ALG_BYTES_PER_CELL = 128.0
+TRAFFIC_PROFILE = os.path.join("profiles", "stencil_traffic_6400x2560.json")
 
 
 def workload_config(L, M):
@@ -37,21 +53,147 @@ def workload_config(L, M):
     return cases.channel_config(L, M, geometry="condiv", noise=1e-3, max_steps=0)
 
 
-def cpu_baseline(L, M, steps):
-    """The oracle on the host cores: IC setup untimed, `steps` timed steps."""
+# ------------------------------------------------------------ CPU baseline
+
+def host_topology():
+    """Physical cores of the CPUs this process may run on (lscpu's core and
+    socket ids), with the machine totals."""
+    cpus = sorted(os.sched_getaffinity(0))
+    info = {"affinity_cpus": len(cpus), "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu", "-p=cpu,core,socket"], capture_output=True, text=True, check=True).stdout
+        rows = [tuple(int(v) for v in ln.split(",")) for ln in out.splitlines() if ln and not ln.startswith("#")]
+        allowed = set(cpus)
+        cores = {(c, s) for cpu, c, s in rows if cpu in allowed}
+        info.update(physical_cores=len(cores), sockets=len({s for _, _, s in rows}),
+                    machine_physical_cores=len({(c, s) for _, c, s in rows}))
+        model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for ln in model.splitlines():
+            if ln.startswith("Model name:"):
+                info["model"] = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        info["physical_cores"] = len(cpus)
+    return info
+
+
+def _oracle_child(L, M, steps, threads):
+    """Runs the oracle in a child process whose OpenMP runtime starts with
+    the given threads, bound close, one place per core (the reference's
+    slurm.sh:11-14 settings).  Returns the child's JSON."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.abspath(__file__), "--mode", "cpu-oracle-run", "--L", str(L), "--M", str(M),
+           "--cpu-steps", str(steps)]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, check=True).stdout
+    return json.loads(out.strip().splitlines()[-1])
+
+
+def cpu_oracle_run(L, M, steps):
+    """(child) the oracle on the workload: IC setup and one warm-up step
+    untimed, `steps` timed steps."""
     import _oracle as O
     import cases
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = O.Oracle(cases.oracle_case(workload_config(L, M)))
-    o.run(1)  # warm-up step (first touch of the arrays)
+    o.run(1)
     t0 = time.perf_counter()
     o.run(steps)
     dt = time.perf_counter() - t0
-    cells = L * (M + 1)
-    return {"value": cells * steps / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of the full {L}x{M} workload after 1 warm-up step "
-                      f"({dt:.1f} s), oracle/lf_oracle.c with OpenMP"}
+    print(json.dumps({"seconds": dt, "steps": steps, "gcups": L * (M + 1) * steps / dt / 1e9,
+                      "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}), flush=True)
 
+
+REF_HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+
+
+def _ref_child(L, M, steps, threads):
+    """The reference's own compiled step (oracle/_ref/ref_harness, timing
+    mode) with OpenMP pinned as slurm.sh:11-14 pins it."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = subprocess.run([REF_HARNESS, "time", str(L), str(M), str(steps)], env=env, capture_output=True,
+                         text=True, check=True).stdout
+    kv = dict(ln.split(None, 1) for ln in out.splitlines() if ln.strip())
+    secs = float(kv["loop_seconds"])
+    return {"seconds": secs, "steps": steps, "gcups": L * (M + 1) * steps / secs / 1e9}
+
+
+def cpu_baseline(L, M, steps, steps_1t, ref_steps, ref_steps_1t):
+    """The reference's own step (when oracle/_ref/ref_harness was built with
+    the tree) and the oracle port, each with all physical cores of this
+    process (at most 16) and with one thread."""
+    topo = host_topology()
+    threads = max(1, min(topo.get("physical_cores", 1), 16))
+    pin = "OMP_PROC_BIND=close OMP_PLACES=cores"
+    port = {}
+    many = _oracle_child(L, M, steps, threads)
+    port = {"value": many["gcups"], "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the full {L}x{M} workload after 1 untimed step ({many['seconds']:.1f} s), "
+                      f"oracle/lf_oracle.c (six passes per step, as src/solver.cpp:22-37) with OpenMP, "
+                      f"{threads} threads, {pin}"}
+    if steps_1t > 0:
+        one = _oracle_child(L, M, steps_1t, 1)
+        port["one_thread"] = {"value": one["gcups"], "unit": "GCUPS",
+                              "sample": f"{steps_1t} steps after 1 untimed step ({one['seconds']:.1f} s)"}
+    if not os.path.exists(REF_HARNESS):
+        port["host"] = topo
+        return port
+    ref = _ref_child(L, M, ref_steps, threads)
+    out = {"value": ref["gcups"], "unit": "GCUPS", "cores": threads, "kind": "reference",
+           "sample": f"{ref_steps} steps of the {L}x{M} workload after 1 untimed step ({ref['seconds']:.1f} s): "
+                     f"the reference's Solver::Advance + Diagnostics::ComputeDt compiled from its own sources "
+                     f"(oracle/_ref/ref_harness, timing mode; no boundary segments, whose O(L+M) work per step "
+                     f"is left out), {threads} threads, {pin}",
+           "host": topo, "port": port}
+    if ref_steps_1t > 0:
+        one = _ref_child(L, M, ref_steps_1t, 1)
+        out["one_thread"] = {"value": one["gcups"], "unit": "GCUPS",
+                             "sample": f"{ref_steps_1t} steps after 1 untimed step ({one['seconds']:.1f} s)"}
+    return out
+
+
+def cpu_calibration():
+    """The oracle against the reference's own compiled solver (oracle/_ref/
+    ref_harness, no boundary segments: the state the harness can run) on the
+    same state and threads: loop wall time of each, ratio oracle / reference."""
+    import numpy as np
+    import _oracle as O
+    if not O.ref_harness_available():
+        sys.exit("oracle/_ref/ref_harness is missing (build it with make -C oracle ref where the reference is)")
+    topo = host_topology()
+    threads = int(os.environ.get("OMP_NUM_THREADS", topo.get("physical_cores", 1)))
+    ic = {"rho": "1.0", "v_z": "0.1", "v_r": "0.0", "v_phi": "0.0", "H_z": "0.0", "H_r": "0.0",
+          "H_phi": "0.2 / r", "e": "0.5 / ((gamma - 1) * rho)"}
+    rows = []
+    for L, M, steps, reps in ((200, 80, 10, 5), (1600, 640, 10, 3)):
+        gp = [0.2, 0.05, 0.8, 0.1]
+        case = dict(L=L, M=M, gamma=1.6666666666666667, geom=(O.GEOM_CONDIV, gp), ic=ic, cfl=0.4, growth=1.1,
+                    dt_min=1e-12, dt_max=1e-2, dt=1e-4, adaptive=True)
+        o0 = O.Oracle(case)
+        phys = [o0.phys(k).copy() for k in range(10)]
+        u0 = [o0.u0(k).copy() for k in range(8)]
+        ref_t, orc_t = [], []
+        for _ in range(reps):
+            out = O.run_ref_harness(L, M, O.GEOM_CONDIV, gp, 1.6666666666666667, steps, 1e-4, 1, 0.4, 1.1, 1e-12,
+                                    1e-2, phys, u0)
+            ref_t.append(out["loop_seconds"])
+            o = O.Oracle(case)
+            for k in range(10):
+                o.phys(k)[:] = phys[k]
+            for k in range(8):
+                o.u0(k)[:] = u0[k]
+            dt = 1e-4
+            t0 = time.perf_counter()
+            for _s in range(steps):
+                o.advance(dt)
+                dt = o.compute_dt(dt)
+            orc_t.append(time.perf_counter() - t0)
+            same = all(np.array_equal(np.nan_to_num(o.u0(k)), np.nan_to_num(out["u0"][k])) for k in range(8))
+        r, c = min(ref_t), min(orc_t)
+        rows.append({"grid": f"{L}x{M}", "steps": steps, "reps": reps, "reference_s": r, "oracle_s": c,
+                     "oracle_over_reference": c / r, "same_state": bool(same)})
+    print(json.dumps({"threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+                      "omp_places": os.environ.get("OMP_PLACES"), "host": topo, "runs": rows}, indent=1))
+
+
+# ------------------------------------------------------------- GPU extras
 
 def sweep_measure(steps=1000, warm=200, n=64, seed=12345):
     """Batched sweep at the uneven sizes nz in {200, 400, 800, 1600}, nr = 0.4 nz
@@ -114,38 +256,30 @@ def warmstart_measure(train, test, epochs, L=96, M=39):
     keep = ["plain_cold_total_s", "cold_batch_s", "warm_total_s", "speedup_vs_plain_cold", "speedup_vs_cold_batch",
             "steps_ratio_cold_over_warm", "fallbacks", "all_converged_warm", "warm_finish_p50_s",
             "warm_finish_p99_s", "max_field_err_warm_vs_cold"]
-    out = {k: r[k] for k in keep}
+    out = {k: r[k] for k in keep if k in r}
     out.update({"workload": f"{test} held-out sweep cases at {L}x{M}, residual 1e-8; network trained on "
                             f"{int(conv.sum())} steady states, {epochs} epochs",
                 "generate_s": round(gen_s, 2), "train_s": round(train_s, 2), "final_loss": losses[-1]})
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--L", type=int, default=6400)
-    ap.add_argument("--M", type=int, default=2560)
-    ap.add_argument("--cpu-steps", type=int, default=80)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tile-order", type=int, default=None,
-                    help="stencil tile order (library default 0); for profiling the alternatives")
-    ap.add_argument("--tile-rows", type=int, default=None, help="stencil tile height (library default 8)")
-    ap.add_argument("--no-sweep", action="store_true")
-    ap.add_argument("--warmstart-train", type=int, default=500)
-    ap.add_argument("--warmstart-test", type=int, default=64)
-    ap.add_argument("--warmstart-epochs", type=int, default=60)
-    ap.add_argument("--no-warmstart", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU z-slab decomposition is design-only).", file=sys.stderr)
-        sys.exit(2)
+def traffic_record():
+    """HBM bytes per stencil launch from the committed counter profile (two
+    rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy
+    kernel's calibration), with its source named."""
+    path = os.path.join(REPO, TRAFFIC_PROFILE)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec.get("bytes_per_launch"), {"file": TRAFFIC_PROFILE, "measured": rec.get("measured", ""),
+                                         "kernel": rec.get("kernel", "")}
 
+
+# ------------------------------------------------------------------ modes
+
+def mode_step(args):
     import mpd_amd
-
     L, M = args.L, args.M
     cells = L * (M + 1)
     print(f"bench: {L}x{M} converging-diverging channel, {cells} cells, warmup {args.warmup}, "
@@ -160,12 +294,10 @@ def main():
         wall = time.perf_counter() - w0
     ms_per_step = tm["total_ms"] / args.steps
     gcups = cells / (ms_per_step * 1e-3) / 1e9
-    achieved = ALG_BYTES_PER_CELL * cells / (tm["stencil_ms"] * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(REPO, "profiles", "stencil_traffic_6400x2560.json")
-    if os.path.exists(tf) and (L, M) == (6400, 2560):
-        with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    alg = ALG_BYTES_PER_CELL * cells
+    achieved = alg / (ms_per_step * 1e-3) / 1e9
+    k_achieved = alg / (tm["stencil_ms"] * 1e-3) / 1e9
+    traffic, tsrc = traffic_record() if (L, M) == (6400, 2560) else (None, None)
     line = {
         "metric": "GCUPS (FP64 LF step, 6400x2560 converging-diverging channel)",
         "value": round(gcups, 3),
@@ -182,13 +314,23 @@ def main():
         "config": {"workload": f"converging-diverging channel {L}x{M}, FP64 LF, adaptive CFL 0.4",
                    "cells": cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "definition": "128 B x cells x steps / wall time of the timed steps (whole step)",
+                     "traffic_source": tsrc},
+        "kernel_roofline": {"kernel": "lf_stencil_tiled", "achieved": round(k_achieved, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(k_achieved / HBM_PEAK_GBS, 4),
+                            "launch_ms": round(tm["stencil_ms"], 5)},
         "step_breakdown_ms": {"stencil": round(tm["stencil_ms"], 5), "boundary": round(tm["bc_ms"], 5)},
         "wall_s": round(wall, 3),
     }
     for k in ("tile_order", "tile_rows"):
         if getattr(args, k) is not None:
             line["config"][k] = getattr(args, k)
+    if not args.no_copy:
+        c = mpd_amd.bench_copy(1024, 0)
+        line["hbm_copy_ceiling"] = {"gbs": round(c["ceiling_gbs"], 1), "frac_of_peak":
+                                    round(c["ceiling_gbs"] / HBM_PEAK_GBS, 4),
+                                    "shape": f"{c['best_blocks']} workgroups, {c['best_stores']} stores, 1 GiB buffers"}
     if not args.no_sweep:
         print("bench: batched sweep", file=sys.stderr, flush=True)
         line["batched_sweep"] = sweep_measure()
@@ -197,8 +339,116 @@ def main():
         line["steady_state"] = warmstart_measure(args.warmstart_train, args.warmstart_test, args.warmstart_epochs)
     if not args.no_cpu:
         print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(L, M, args.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(L, M, args.cpu_steps, args.cpu_steps_1t, args.ref_steps,
+                                            args.ref_steps_1t)
     print(json.dumps(line), flush=True)
+
+
+def mode_copy(args):
+    import mpd_amd
+    c = mpd_amd.bench_copy(args.copy_mib, 3)
+    print(json.dumps(c), flush=True)
+
+
+def mode_sweep_probe(args):
+    import mpd_amd
+    import sweep
+    with mpd_amd.Batch(sweep.probe_configs(args.probe)) as b:
+        b.run(args.warmup)
+        print(b.autotune(5), flush=True)
+        t0 = time.perf_counter()
+        b.run(args.steps)
+        el = time.perf_counter() - t0
+    print(f"{args.probe}: {args.steps} steps {el:.3f} s = {el / args.steps * 1e6:.1f} us/step", flush=True)
+
+
+def mode_sweep_converge(args):
+    import mpd_amd
+    import sweep
+    r = sweep.converge_uneven(mpd_amd, [int(v) for v in args.sizes.split(",")])
+    print(json.dumps(r, indent=1), flush=True)
+
+
+def mode_warmstart(args):
+    import warmstart
+    warmstart.experiment(args.L, args.M, args.train, args.test, args.epochs, args.batch, args.lr, args.clip,
+                         args.plain_sample, args.seed, args.out)
+
+
+def mode_nn_train(args):
+    import surrogate
+    ms = surrogate.time_training(args.H, args.W, args.batch, args.steps)
+    print(f"training step, batch {args.batch} at {args.H}x{args.W}: {ms:.2f} ms", flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", default="step",
+                    choices=["step", "copy", "sweep-probe", "sweep-converge", "warmstart", "nn-train",
+                             "cpu-calibration", "cpu-oracle-run"])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--L", type=int, default=None)
+    ap.add_argument("--M", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps-1t", type=int, default=4)
+    ap.add_argument("--ref-steps", type=int, default=40)
+    ap.add_argument("--ref-steps-1t", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--tile-order", type=int, default=None,
+                    help="stencil tile order (library default 0); for profiling the alternatives")
+    ap.add_argument("--tile-rows", type=int, default=None, help="stencil tile height (library default 8)")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--warmstart-train", type=int, default=500)
+    ap.add_argument("--warmstart-test", type=int, default=64)
+    ap.add_argument("--warmstart-epochs", type=int, default=60)
+    ap.add_argument("--no-warmstart", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--probe", choices=["small", "uneven"], default="small")
+    ap.add_argument("--sizes", default="48,96,144,192")
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--test", type=int, default=64)
+    ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--clip", type=float, default=None)
+    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--H", type=int, default=96)
+    ap.add_argument("--W", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU z-slab decomposition is design-only).", file=sys.stderr)
+        sys.exit(2)
+    m = args.mode
+    if m in ("step", "cpu-oracle-run"):
+        args.L = args.L or 6400
+        args.M = args.M or 2560
+    if m == "step":
+        args.steps = args.steps or 10000
+        mode_step(args)
+    elif m == "cpu-oracle-run":
+        cpu_oracle_run(args.L, args.M, args.cpu_steps)
+    elif m == "cpu-calibration":
+        cpu_calibration()
+    elif m == "copy":
+        mode_copy(args)
+    elif m == "sweep-probe":
+        args.steps = args.steps or 2000
+        mode_sweep_probe(args)
+    elif m == "sweep-converge":
+        mode_sweep_converge(args)
+    elif m == "warmstart":
+        args.L = args.L or 96
+        args.M = args.M or 39
+        mode_warmstart(args)
+    elif m == "nn-train":
+        args.steps = args.steps or 40
+        mode_nn_train(args)
 
 
 if __name__ == "__main__":

@@ -144,6 +144,16 @@ int mpd_set_option(mpd_solver* s, const char* key, double value);
 int mpd_time_steps(mpd_solver* s, int steps, int warmup, double* total_ms, double* stencil_ms,
                    double* bc_ms);
 
+/* Benchmark helper, no counterpart in the reference: copies between two
+ * device buffers of `mib` MiB each.  First `calib_launches` launches of one
+ * fixed shape (2048 workgroups of 256, default stores; each reads and writes
+ * exactly mib MiB: the calibration of the HBM traffic counters), their mean
+ * time in calib_ms; then a sweep over workgroup counts and store policy
+ * whose best rate (bytes read + written per second) is the copy ceiling.
+ * Any output pointer may be NULL. */
+int mpd_bench_copy(long long mib, int calib_launches, double* calib_ms, double* ceiling_gbs, int* best_blocks,
+                   int* best_nt);
+
 /* ------------------------------------------------------------------------
  * Batched parameter sweeps.  No counterpart in the reference, which runs one
  * case per process (src/main.cpp:22-317); a sweep over inflow parameters,

@@ -42,7 +42,7 @@ EXPORTS = ["mpd_create_from_file", "mpd_create_from_string", "mpd_destroy", "mpd
            "mpd_config_check", "mpd_initial_state", "mpd_set_option", "mpd_time_steps", "mpd_probe",
            "mpd_write_vtk", "mpd_write_vtk_fields", "mpd_batch_create", "mpd_batch_destroy", "mpd_batch_size",
            "mpd_batch_instance", "mpd_batch_set_option", "mpd_batch_autotune", "mpd_batch_tile_rows",
-           "mpd_batch_run"]
+           "mpd_batch_run", "mpd_bench_copy"]
 
 _lib = None
 _DP = C.POINTER(C.c_double)
@@ -88,6 +88,7 @@ def load():
     L.mpd_batch_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_int]
     L.mpd_batch_tile_rows.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.mpd_batch_run.argtypes = [vp, C.c_int, C.POINTER(RunResult)]
+    L.mpd_bench_copy.argtypes = [C.c_longlong, C.c_int, _DP, _DP, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     for name in EXPORTS:
         if name not in ("mpd_destroy", "mpd_last_error", "mpd_batch_destroy"):
             getattr(L, name).restype = C.c_int
@@ -119,6 +120,17 @@ def initial_state(config):
     pa = [np.zeros((rows, cols)) for _ in range(10)]
     _check(load().mpd_initial_state(config.encode(), _ptr_array(ua), _ptr_array(pa)))
     return ua, pa
+
+
+def bench_copy(mib=1024, calib_launches=3):
+    """Copy ceiling of the GPU's HBM (mpd_bench_copy): dict with the mean
+    time of the calibration launches and the best copy rate in GB/s."""
+    a, b = C.c_double(), C.c_double()
+    blocks, nt = C.c_int(), C.c_int()
+    _check(load().mpd_bench_copy(int(mib), int(calib_launches), C.byref(a), C.byref(b), C.byref(blocks),
+                                 C.byref(nt)))
+    return dict(mib=mib, calib_ms=a.value, ceiling_gbs=b.value, best_blocks=blocks.value,
+                best_stores="non-temporal" if nt.value else "default")
 
 
 def write_vtk_fields(config, phys, path, binary=True):

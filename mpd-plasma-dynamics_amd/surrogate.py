@@ -533,24 +533,18 @@ def u0_from_fields(f, r, rows, cols):
     return out
 
 
-def main():
-    """Times training steps (forward, MSE, backward, Adam) on random data."""
-    import argparse
+def time_training(H, W, B, steps):
+    """Mean milliseconds of one training step (forward, MSE, backward, Adam)
+    of the network at batch B on an H x W grid, random data, after up to 10
+    untimed steps."""
     import time
-    ap = argparse.ArgumentParser(description="time training steps of the warm-start network")
-    ap.add_argument("--H", type=int, default=96)
-    ap.add_argument("--W", type=int, default=40)
-    ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--steps", type=int, default=40)
-    args = ap.parse_args()
-    H, W, B = args.H, args.W, args.batch
     net = UNet(H, W, seed=0)
     rng = np.random.default_rng(0)
-    x = DeviceArray.from_numpy(rng.normal(size=(B, H, W, N_FEATURES)), "bf16")
-    t = DeviceArray.from_numpy(rng.normal(size=(B, H, W, len(OUT_FIELDS))))
-    dy = DeviceArray((B, H, W, len(OUT_FIELDS)))
-    warm = min(10, args.steps - 1)
-    for it in range(args.steps):
+    x = DeviceArray.from_numpy(rng.normal(size=(B, net.H, net.W, N_FEATURES)), "bf16")
+    t = DeviceArray.from_numpy(rng.normal(size=(B, net.H, net.W, len(OUT_FIELDS))))
+    dy = DeviceArray((B, net.H, net.W, len(OUT_FIELDS)))
+    warm = min(10, steps - 1)
+    for it in range(steps):
         if it == warm:
             _ck(nn_lib().mpd_nn_synchronize())
             t0 = time.perf_counter()
@@ -559,9 +553,4 @@ def main():
         net.backward(dy, B)
         net.adam_step(1e-3)
     _ck(nn_lib().mpd_nn_synchronize())
-    print(f"training step, batch {B} at {H}x{W}: {(time.perf_counter() - t0) / (args.steps - warm) * 1e3:.2f} ms",
-          flush=True)
-
-
-if __name__ == "__main__":
-    main()
+    return (time.perf_counter() - t0) / (steps - warm) * 1e3

@@ -189,36 +189,3 @@ def converge_uneven(mpd_amd, sizes, n=64, seed=2, threshold=1e-8, plain_sample=1
             "one_at_a_time_mean_s_by_size": mean, "speedup": plain_s / batch_s,
             "steps_p50": float(np.median(steps)), "steps_max": int(steps.max()),
             "finish_p50_s": float(np.nanmedian(finish)), "finish_p99_s": float(np.nanpercentile(finish, 99))}
-
-
-def main():
-    import argparse
-    import json
-    import os
-    import sys
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    import mpd_amd
-    ap = argparse.ArgumentParser(description="time a batched sweep workload")
-    ap.add_argument("--probe", choices=["small", "uneven"])
-    ap.add_argument("--converge", type=str, default="",
-                    help="comma-separated nz sizes: run 64 uneven cases to residual 1e-8, batched and one at a time")
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
-    args = ap.parse_args()
-    if args.converge:
-        r = converge_uneven(mpd_amd, [int(v) for v in args.converge.split(",")])
-        print(json.dumps(r, indent=1), flush=True)
-        return
-    if not args.probe:
-        ap.error("--probe or --converge is required")
-    with mpd_amd.Batch(probe_configs(args.probe)) as b:
-        b.run(args.warmup)
-        print(b.autotune(5), flush=True)
-        t0 = time.perf_counter()
-        b.run(args.steps)
-        el = time.perf_counter() - t0
-    print(f"{args.probe}: {args.steps} steps {el:.3f} s = {el / args.steps * 1e6:.1f} us/step", flush=True)
-
-
-if __name__ == "__main__":
-    main()

@@ -4,10 +4,8 @@ the warm-versus-cold comparison in which the plain LF loop is both the arbiter
 (each warm start is relaxed to the same convergence threshold as a cold
 start) and the fallback (a case whose residual stalls starts again cold).
 
-Run as a script for the full experiment:
-  python warmstart.py --train 2000 --test 64 --L 96 --M 39 --out DIR
+The full experiment is `experiment` (bench.py --mode warmstart).
 """
-import argparse
 import json
 import os
 import sys
@@ -368,26 +366,16 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     }, warm_fields, cold_fields
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=2000)
-    ap.add_argument("--test", type=int, default=64)
-    ap.add_argument("--L", type=int, default=96)
-    ap.add_argument("--M", type=int, default=39)
-    ap.add_argument("--epochs", type=int, default=150)
-    ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--clip", type=float, default=None, help="optional global gradient-norm clip")
-    ap.add_argument("--plain-sample", type=int, default=16)
-    ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--out", default="")
-    args = ap.parse_args()
-    L, M = args.L, args.M
+def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, clip=None, plain_sample=16,
+               seed=2024, out=""):
+    """The whole experiment: train_n steady states generated, the network
+    trained on them, test_n held-out cases evaluated (evaluate); prints the
+    result as JSON and, with `out`, writes it and the trained weights there."""
     t0 = time.perf_counter()
-    params, fields, steps, conv = generate(args.train, L, M, args.seed)
+    params, fields, steps, conv = generate(train_n, L, M, seed)
     gen_s = time.perf_counter() - t0
     keep = conv
-    progress(f"generated {args.train} steady states in {gen_s:.1f}s ({keep.sum()} converged, steps p50 "
+    progress(f"generated {train_n} steady states in {gen_s:.1f}s ({keep.sum()} converged, steps p50 "
              f"{np.median(steps):.0f} max {steps.max()})")
     params, fields = params[keep], fields[keep]
     radii = grid_radii(params, L, M)
@@ -395,27 +383,24 @@ def main():
     norm = Normalizer(fields - base)
     X = SG.features(params, radii, L, M)
     T = norm.encode(fields - base).astype(np.float32)
-    net = SG.UNet(L, M + 1, seed=args.seed)
+    net = SG.UNet(L, M + 1, seed=seed)
     progress(f"network: {net.n_params()} parameters, {len(net.layers)} convs")
     t1 = time.perf_counter()
     tstats = {}
-    losses = train(net, X, T, args.epochs, args.batch, args.lr, seed=args.seed, clip=args.clip, stats=tstats)
+    losses = train(net, X, T, epochs, batch, lr, seed=seed, clip=clip, stats=tstats)
     train_s = time.perf_counter() - t1
-    test_params = sweep.sample_params(args.test, args.seed + 1)
-    result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=args.plain_sample,
+    test_params = sweep.sample_params(test_n, seed + 1)
+    result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=plain_sample,
                             budget=step_budget(steps[keep]))
-    result.update({"train_cases": int(keep.sum()), "generate_s": gen_s, "train_s": train_s, "epochs": args.epochs,
-                   "clip": args.clip, "rollbacks": tstats["rollbacks"],
+    result.update({"train_cases": int(keep.sum()), "generate_s": gen_s, "train_s": train_s, "epochs": epochs,
+                   "clip": clip, "rollbacks": tstats["rollbacks"],
                    "final_loss": losses[-1], "first_loss": losses[0], "n_params": net.n_params()})
     print(json.dumps(result, indent=1), flush=True)
-    if args.out:
-        os.makedirs(args.out, exist_ok=True)
-        with open(os.path.join(args.out, "warmstart_result.json"), "w") as f:
+    if out:
+        os.makedirs(out, exist_ok=True)
+        with open(os.path.join(out, "warmstart_result.json"), "w") as f:
             json.dump(result, f, indent=1)
         sd = net.state_dict()
-        np.savez(os.path.join(args.out, "warmstart_net.npz"), **{f"w{k}": w for k, w in enumerate(sd["w"])},
+        np.savez(os.path.join(out, "warmstart_net.npz"), **{f"w{k}": w for k, w in enumerate(sd["w"])},
                  **{f"b{k}": b for k, b in enumerate(sd["b"])}, mean=norm.mean, std=norm.std)
-
-
-if __name__ == "__main__":
-    main()
+    return result

@@ -530,10 +530,12 @@ static void apply_face(orc_solver* s, int face, double dt) {
     }
 }
 
-/* Fields::CopyUToU0, src/fields.cpp:102-116. */
+/* Fields::CopyUToU0, src/fields.cpp:102-116 (OpenMP over rows, as there). */
 static void copy_u_to_u0(orc_solver* s) {
-    const size_t n = (size_t)s->rows * (size_t)s->cols;
-    for (int k = 0; k < 8; ++k) memcpy(s->u0[k], s->u[k], n * sizeof(double));
+    const size_t cols = (size_t)s->cols;
+#pragma omp parallel for
+    for (int l = 0; l < s->rows; ++l)
+        for (int k = 0; k < 8; ++k) memcpy(s->u0[k] + l * cols, s->u[k] + l * cols, cols * sizeof(double));
 }
 
 /* Solver::Advance, src/solver.cpp:22-37: central update, central physical,

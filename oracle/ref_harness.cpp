@@ -18,6 +18,18 @@
 // in.bin : 10 physical arrays (rho v_z v_r v_phi e p P H_z H_r H_phi) then the
 //          8 u0 arrays, each (L_max+2) x (M_max+3) doubles, row-major.
 // out.bin: see the write order at the end of main().
+//
+// Timing mode: ref_harness time L_max M_max steps builds the memory-bound
+// channel workload in the process (converging-diverging walls r_in = 0.2 +
+// 0.05 sin(pi z), r_out = 0.8 - 0.1 z^2; rho = 1 with +-1e-3 SplitMix64
+// noise of seed 0, v_z = 0.1, H_phi = 0.2 / r, p = 0.5; gamma 5/3, adaptive
+// CFL 0.4), runs one untimed step and then `steps` steps of the time loop
+// (Solver::Advance + Diagnostics::ComputeDt), and prints the loop's wall
+// time: the reference's own step timed as the CPU baseline.  No boundary
+// segments (see above): the faces' O(L + M) work per step is left out.
+#include <sched.h>
+
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -86,7 +98,79 @@ void GetArray(std::FILE* f, Array2D& a) {
 
 }  // namespace
 
+namespace {
+
+// SplitMix64 (Steele, Lea, Flood 2014) of (seed, idx) as a uniform in [0, 1).
+double NoiseU01(unsigned long long seed, unsigned long long idx) {
+    unsigned long long z = seed + (idx + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+int TimeMain(int argc, char** argv) {
+    SimConfig cfg;
+    cfg.L_max = std::atoi(argv[2]);
+    cfg.M_max = std::atoi(argv[3]);
+    const int steps = std::atoi(argv[4]);
+    if (cfg.L_max < 2 || cfg.M_max < 2 || steps < 1) return 2;
+    cfg.gamma = 1.6666666666666667;
+    cfg.dt = 1.0e-4;
+    cfg.adaptive_dt = true;
+    cfg.cfl_number = 0.4;
+    cfg.dt_growth_factor = 1.1;
+    cfg.dt_min = 1.0e-12;
+    cfg.dt_max = 1.0e-2;
+    cfg.dz = 1.0 / cfg.L_max;
+    cfg.dy = 1.0 / cfg.M_max;
+    cfg.mpi_dims_l = 1;
+    cfg.mpi_dims_m = 1;
+    cfg.convergence_threshold = 1.0;
+    int ac = 1;
+    MPIManager mpi(ac, argv, cfg);
+    const double gp[5] = {0.2, 0.05, 0.8, 0.1, 0.0};
+    HarnessGeometry geom(1, gp);
+    Grid grid(cfg, mpi.local_L_with_ghosts, mpi.l_start, mpi.local_M_with_ghosts, mpi.m_start, geom);
+    Fields f(mpi.local_L_with_ghosts, mpi.local_M_with_ghosts, true);
+    const int Mn = cfg.M_max + 1;
+    const double gm1 = cfg.gamma - 1.0;
+    for (int l = 1; l <= cfg.L_max; ++l) {
+        for (int m = 1; m <= Mn; ++m) {
+            const double r = grid.r[l][m];
+            const double u = NoiseU01(0, (unsigned long long)(l - 1) * Mn + (m - 1));
+            const double rho = 1.0 * (1.0 + 1.0e-3 * (2.0 * u - 1.0));
+            f.rho[l][m] = rho;
+            f.v_z[l][m] = 0.1;
+            f.H_phi[l][m] = 0.2 / r;
+            f.e[l][m] = 0.5 / (gm1 * rho);
+            f.p[l][m] = gm1 * rho * f.e[l][m];
+            f.P[l][m] = f.p[l][m] + 0.5 * (f.H_phi[l][m] * f.H_phi[l][m]);
+        }
+    }
+    f.InitConservative(grid);
+    Solver solver(cfg, mpi, grid, f);
+    double dt = cfg.dt;
+    solver.Advance(dt);
+    dt = Diagnostics::ComputeDt(f, cfg, mpi.local_L, mpi.local_M, mpi, dt);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < steps; ++s) {
+        solver.Advance(dt);
+        dt = Diagnostics::ComputeDt(f, cfg, mpi.local_L, mpi.local_M, mpi, dt);
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    const int aff = sched_getaffinity(0, sizeof(set), &set) == 0 ? CPU_COUNT(&set) : -1;
+    std::printf("loop_seconds %.9f\nsteps %d\ndt %.17g\naffinity_cpus %d\n", secs, steps, dt, aff);
+    (void)argc;
+    return 0;
+}
+
+}  // namespace
+
 int main(int argc, char** argv) {
+    if (argc == 5 && std::string(argv[1]) == "time") return TimeMain(argc, argv);
     if (argc != 19) {
         std::fprintf(stderr, "usage: ref_harness L M geom p0 p1 p2 p3 p4 gamma steps dt "
                              "adaptive cfl growth dt_min dt_max in out\n");
@@ -153,6 +237,7 @@ int main(int argc, char** argv) {
     Solver solver(cfg, mpi, grid, f);
     double dt = cfg.dt;
     std::vector<double> dts;
+    const auto t0 = std::chrono::steady_clock::now();
     for (int s = 0; s < steps; ++s) {
         solver.Advance(dt);
         dts.push_back(dt);
@@ -160,6 +245,9 @@ int main(int argc, char** argv) {
             dt = Diagnostics::ComputeDt(f, cfg, mpi.local_L, mpi.local_M, mpi, dt);
         }
     }
+    // Wall time of the loop alone (the CPU-baseline calibration reads it).
+    std::fprintf(stderr, "loop_seconds %.9f\n",
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
 
     for (auto* a : u0) PutArray(out, *a);
     for (auto* a : phys) PutArray(out, *a);

@@ -281,8 +281,9 @@ def run_ref_harness(L, M, geom, gp, gamma, steps, dt, adaptive, cfl, growth, dt_
             repr(float(gamma)), str(steps), repr(float(dt)), str(int(adaptive)), repr(float(cfl)),
             repr(float(growth)), repr(float(dt_min)), repr(float(dt_max)), fin, fout]
         env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "4"))
-        subprocess.run(args, check=True, env=env, stdout=subprocess.DEVNULL)
+        pr = subprocess.run(args, check=True, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
         raw = np.fromfile(fout, dtype=np.float64)
+    loop_s = [float(x.split()[1]) for x in pr.stderr.splitlines() if x.startswith("loop_seconds")]
     n2 = rows * cols
     o = {}
     pos = 0
@@ -309,5 +310,6 @@ def run_ref_harness(L, M, geom, gp, gamma, steps, dt, adaptive, cfl, growth, dt_
     o["integrals"] = take(7)
     o["mass_flux"] = take(1)[0]
     o["thrust"] = take(1)[0]
+    o["loop_seconds"] = loop_s[0] if loop_s else None
     assert pos == raw.size, (pos, raw.size)
     return o
