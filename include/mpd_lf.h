@@ -103,6 +103,16 @@ int mpd_radial_symmetry(mpd_solver* s, double* out);
  * R and dr as rows entries, r_0.  Any pointer may be NULL. */
 int mpd_grid(const mpd_solver* s, double* r, double* r_z, double* R, double* dr, double* r_0);
 
+/* No counterpart in the reference: mpd_download for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arrays, same bits; the physical
+ * arrays are staged through the library's fixed per-GPU scratch array
+ * (16 MiB, held until the process ends) a chunk of rows at a time instead
+ * of a buffer allocated for the call.  MPD_E_ARG when one row of the 10
+ * physical fields does not fit the scratch (M_max above about 209,000).
+ * Calls on one GPU share its scratch; the library serialises them. */
+int mpd_download_noalloc(mpd_solver* s, double* const* u0, double* const* phys);
+
 /* Host-only (no GPU needed): parse and validate a config (SimConfig::Load,
  * geometry, IC and BC compilation) and report the grid sizes. */
 int mpd_config_check(const char* yaml_text, int* L_max, int* M_max);

@@ -1264,6 +1264,27 @@ __global__ void __launch_bounds__(256) lf_phys_fields(StepArgs a, double* out) {
     }
 }
 
+// The same physical fields for the rows r0 .. r0 + nrows - 1 only (ghost
+// rows and columns 0), into out: field k's row r0 + i at
+// out[(k * nrows + i) * pitch], in the strip layout of the state.
+__global__ void __launch_bounds__(256) lf_phys_rows(StepArgs a, int r0, int nrows, double* out) {
+    const int cols = a.Mn + 2;
+    const int64_t n = (int64_t)nrows * cols;
+    const int64_t fs = (int64_t)nrows * a.pitch;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / cols), m = (int)(i % cols);
+        const int l = r0 + r;
+        double ph[10];
+        if (l >= 1 && l <= a.L && m >= 1 && m <= a.Mn) {
+            start_phys(a, l, m, ph);
+        } else {
+            for (int k = 0; k < 10; ++k) ph[k] = 0.0;
+        }
+        const int64_t j = (int64_t)r * a.pitch + phys_col(m);
+        for (int k = 0; k < 10; ++k) out[(int64_t)k * fs + j] = ph[k];
+    }
+}
+
 // Verification::ComputeIntegrals (src/verification.cpp:16-45): per-row sums
 // in m order, rows combined in l order by the last block.  rowsum: 7*(L+2).
 __global__ void __launch_bounds__(64) lf_integrals(StepArgs a, double* rowsum, double* out) {
@@ -1499,6 +1520,14 @@ hipError_t launch_max_speed(const StepArgs& a, unsigned long long* out, hipStrea
 
 hipError_t launch_phys_fields(const StepArgs& a, double* out, hipStream_t s) {
     hipLaunchKernelGGL(lf_phys_fields, dim3(1024), dim3(256), 0, s, a, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_phys_rows(const StepArgs& a, int r0, int nrows, double* out, hipStream_t s) {
+    const long long n = (long long)nrows * (a.Mn + 2);
+    const long long blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(lf_phys_rows, dim3((unsigned)(blocks < 1024 ? blocks : 1024)), dim3(256), 0, s, a, r0, nrows,
+                       out);
     return hipGetLastError();
 }
 

@@ -71,6 +71,9 @@ inline int stencil_tiles(int L, int Mn, int H) { return stencil_strips(Mn) * ((L
 inline int default_tile_rows() { return 8; }
 hipError_t launch_max_speed(const StepArgs& a, unsigned long long* out, hipStream_t s);
 hipError_t launch_phys_fields(const StepArgs& a, double* out, hipStream_t s);
+// Physical fields of rows r0 .. r0 + nrows - 1 (10 fields of nrows pitched
+// rows each, field k at out + k * nrows * pitch).
+hipError_t launch_phys_rows(const StepArgs& a, int r0, int nrows, double* out, hipStream_t s);
 hipError_t launch_integrals(const StepArgs& a, double* rowsum, double* out, hipStream_t s);
 hipError_t launch_outlet(const StepArgs& a, int M_max, double* out, hipStream_t s);
 hipError_t launch_radial_sym(const StepArgs& a, unsigned long long* out, hipStream_t s);

@@ -42,7 +42,7 @@ EXPORTS = ["mpd_create_from_file", "mpd_create_from_string", "mpd_destroy", "mpd
            "mpd_config_check", "mpd_initial_state", "mpd_set_option", "mpd_time_steps", "mpd_probe",
            "mpd_write_vtk", "mpd_write_vtk_fields", "mpd_batch_create", "mpd_batch_destroy", "mpd_batch_size",
            "mpd_batch_instance", "mpd_batch_set_option", "mpd_batch_autotune", "mpd_batch_tile_rows",
-           "mpd_batch_run", "mpd_bench_copy"]
+           "mpd_batch_run", "mpd_bench_copy", "mpd_download_noalloc"]
 
 _lib = None
 _DP = C.POINTER(C.c_double)
@@ -65,6 +65,7 @@ def load():
     L.mpd_dims.argtypes = [vp] + [C.POINTER(C.c_int)] * 4
     L.mpd_upload_state.argtypes = [vp, C.POINTER(_DP), C.POINTER(_DP)]
     L.mpd_download.argtypes = [vp, C.POINTER(_DP), C.POINTER(_DP)]
+    L.mpd_download_noalloc.argtypes = [vp, C.POINTER(_DP), C.POINTER(_DP)]
     L.mpd_reset_loop.argtypes = [vp]
     L.mpd_advance.argtypes = [vp, C.c_double]
     L.mpd_compute_dt.argtypes = [vp, C.c_double, _DP]
