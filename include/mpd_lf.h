@@ -47,6 +47,16 @@ typedef struct {
  * (src/main.cpp:28-185; src/solver.cpp:9-16).  The IC state is uploaded. */
 int mpd_create_from_file(const char* yaml_path, mpd_solver** out);
 int mpd_create_from_string(const char* yaml_text, mpd_solver** out);
+
+/* The same with the grid's wall profile given by the caller instead of the
+ * config's geometry section (which is then not read): for a geometry
+ * plug-in that lives in the caller's process.  The four arrays hold, for
+ * l = 0 .. L_max + 1, IGeometry's RInner, ROuter, DrInnerDz and DrOuterDz
+ * (include/igeometry.hpp:16-34) at z = (l - 1) / L_max, the points
+ * Grid::Build evaluates them at (src/grid.cpp:20-42); r_0 is taken from
+ * row l = 1 (z = 0).  Values must be finite (MPD_E_ARG otherwise). */
+int mpd_create_with_grid(const char* yaml_text, const double* r_inner, const double* r_outer,
+                         const double* dr_inner_dz, const double* dr_outer_dz, mpd_solver** out);
 void mpd_destroy(mpd_solver* s);
 const char* mpd_last_error(void);
 

@@ -27,6 +27,20 @@ int mpd_create_from_file(const char* yaml_path, mpd_solver** out) { return Creat
 
 int mpd_create_from_string(const char* yaml_text, mpd_solver** out) { return Create(yaml_text, false, out); }
 
+int mpd_create_with_grid(const char* yaml_text, const double* r_inner, const double* r_outer,
+                         const double* dr_inner_dz, const double* dr_outer_dz, mpd_solver** out) {
+    if (!yaml_text || !out) {
+        g_last_error = "null argument";
+        return MPD_E_ARG;
+    }
+    *out = nullptr;
+    auto s = std::make_unique<mpd_solver>();
+    const HostSetup::RowGeometry rg{r_inner, r_outer, dr_inner_dz, dr_outer_dz};
+    const int rc = Guard([&] { s->Init(yaml_text, false, &rg); });
+    if (rc == MPD_OK) *out = s.release();
+    return rc;
+}
+
 void mpd_destroy(mpd_solver* s) {
     if (s && !s->in_batch) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);

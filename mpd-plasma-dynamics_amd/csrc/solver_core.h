@@ -150,7 +150,16 @@ struct HostSetup {
     MpdBcProgram bc;
     mpd::StencilCfg scfg;
 
-    HostSetup(const std::string& src, bool is_file = false) {
+    // Per-row wall data of a grid computed by the caller (rows = L_max + 2
+    // entries each); when given, the config's geometry section is not used.
+    struct RowGeometry {
+        const double* r_in;
+        const double* r_out;
+        const double* dr_in_dz;
+        const double* dr_out_dz;
+    };
+
+    HostSetup(const std::string& src, bool is_file = false, const RowGeometry* rg = nullptr) {
         mpd::RegisterAllGeometries();
         if (is_file) cfg.Load(src);
         else cfg.LoadFromString(src);
@@ -162,7 +171,6 @@ struct HostSetup {
         else throw std::runtime_error("gpu.kernel must be 'tiled' or 'simple'");
         scfg.fast_div = cfg.gpu.fast_div;
         scfg.tile_rows = cfg.gpu.tile_rows;
-        auto geom = mpd::GeometryRegistry::Instance().Create(cfg.geometry.type, cfg.geometry.params);
         // Initial condition: VTK restart when initial_conditions.vtk_file is
         // set, the expression IC otherwise (src/main.cpp:117-136).
         std::unique_ptr<mpd::IInitialCondition> ic;
@@ -172,7 +180,19 @@ struct HostSetup {
             ic = std::make_unique<mpd::ExpressionIC>(
                 cfg.initial_conditions.has_params ? &cfg.initial_conditions.params : nullptr,
                 cfg.rho_noise_amplitude, cfg.rho_noise_seed);
-        grid.Build(cfg, *geom);
+        if (rg) {
+            const int rows = cfg.L_max + 2;
+            const double* arr[4] = {rg->r_in, rg->r_out, rg->dr_in_dz, rg->dr_out_dz};
+            for (const double* a : arr) {
+                if (!a) throw ArgError("null grid row array");
+                for (int l = 0; l < rows; ++l)
+                    if (!std::isfinite(a[l])) throw ArgError("grid row arrays must be finite");
+            }
+            grid.BuildFromRows(cfg, rg->r_in, rg->r_out, rg->dr_in_dz, rg->dr_out_dz);
+        } else {
+            auto geom = mpd::GeometryRegistry::Instance().Create(cfg.geometry.type, cfg.geometry.params);
+            grid.Build(cfg, *geom);
+        }
         CompileBc(cfg, bc);
         f.Allocate(grid.rows, grid.cols);
         ic->Apply(f, grid, cfg, 0);
@@ -350,8 +370,8 @@ struct mpd_solver {
         HIPCHK(e);
     }
 
-    void Init(const std::string& src, bool is_file) {
-        HostSetup h(src, is_file);
+    void Init(const std::string& src, bool is_file, const HostSetup::RowGeometry* rg = nullptr) {
+        HostSetup h(src, is_file, rg);
         InitFrom(h);
     }
 

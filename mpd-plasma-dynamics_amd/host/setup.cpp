@@ -171,6 +171,29 @@ void HostGrid::Build(const SimConfig& cfg, const IGeometry& g) {
     }
 }
 
+void HostGrid::BuildFromRows(const SimConfig& cfg, const double* r_in, const double* r_out, const double* dr_in_dz,
+                             const double* dr_out_dz) {
+    L = cfg.L_max;
+    M_max = cfg.M_max;
+    Mn = M_max + 1;
+    rows = L + 2;
+    cols = Mn + 2;
+    dz = cfg.dz;
+    dy = cfg.dy;
+    // Grid::Grid's r_0 = (RInner(0) + ROuter(0)) / 2: z = 0 is row l = 1.
+    r_0 = (r_in[1] + r_out[1]) / 2.0;
+    rin.assign(r_in, r_in + rows);
+    rout.assign(r_out, r_out + rows);
+    drin.assign(dr_in_dz, dr_in_dz + rows);
+    drout.assign(dr_out_dz, dr_out_dz + rows);
+    R.assign(rows, 0.0);
+    dr.assign(rows, 0.0);
+    for (int l = 0; l < rows; ++l) {
+        R[l] = rout[l] - rin[l];
+        dr[l] = R[l] / cfg.M_max;
+    }
+}
+
 // --------------------------------------------------------------- fields
 
 void HostFields::Allocate(int r, int c) {

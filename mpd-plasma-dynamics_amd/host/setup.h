@@ -57,6 +57,11 @@ struct HostGrid {
     std::vector<double> rin, rout, drin, drout, R, dr;
 
     void Build(const SimConfig& cfg, const IGeometry& g);
+    // The same grid from per-row wall data computed by the caller (r_in,
+    // r_out and their z-derivatives at z_l = (l - 1) dz, l = 0..L+1: what
+    // Grid::Build evaluates an IGeometry at).
+    void BuildFromRows(const SimConfig& cfg, const double* r_in, const double* r_out, const double* dr_in_dz,
+                       const double* dr_out_dz);
     double r(int l, int m) const {
         const double frac = (m - 1) * dy;
         return (1.0 - frac) * rin[l] + frac * rout[l];

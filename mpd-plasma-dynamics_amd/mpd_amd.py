@@ -36,7 +36,7 @@ class RunResult(C.Structure):
 
 
 # Every symbol include/mpd_lf.h declares.
-EXPORTS = ["mpd_create_from_file", "mpd_create_from_string", "mpd_destroy", "mpd_last_error", "mpd_dims",
+EXPORTS = ["mpd_create_from_file", "mpd_create_from_string", "mpd_create_with_grid", "mpd_destroy", "mpd_last_error", "mpd_dims",
            "mpd_upload_state", "mpd_reset_loop", "mpd_download", "mpd_advance", "mpd_compute_dt", "mpd_max_wave_speed",
            "mpd_solution_change", "mpd_run", "mpd_integrals", "mpd_outlet", "mpd_radial_symmetry", "mpd_grid",
            "mpd_config_check", "mpd_initial_state", "mpd_set_option", "mpd_time_steps", "mpd_probe",
@@ -59,6 +59,7 @@ def load():
     vp = C.c_void_p
     L.mpd_create_from_file.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.mpd_create_from_string.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.mpd_create_with_grid.argtypes = [C.c_char_p, _DP, _DP, _DP, _DP, C.POINTER(vp)]
     L.mpd_destroy.argtypes = [vp]
     L.mpd_destroy.restype = None
     L.mpd_last_error.restype = C.c_char_p
@@ -143,11 +144,16 @@ def write_vtk_fields(config, phys, path, binary=True):
 class Solver:
     """Device-resident LF solver built from a config (path or YAML text)."""
 
-    def __init__(self, config=None, *, path=None, _handle=None, _owner=None):
+    def __init__(self, config=None, *, path=None, grid_rows=None, _handle=None, _owner=None):
+        """grid_rows: optional (r_in, r_out, dr_in_dz, dr_out_dz), each L_max + 2
+        values at z = (l - 1) / L_max, replacing the config's geometry."""
         L = load()
         h = C.c_void_p()
         if _handle is not None:
             h = _handle
+        elif grid_rows is not None:
+            rows = [np.ascontiguousarray(a, dtype=np.float64) for a in grid_rows]
+            _check(L.mpd_create_with_grid(config.encode(), *[a.ctypes.data_as(_DP) for a in rows], C.byref(h)))
         elif path is not None:
             _check(L.mpd_create_from_file(str(path).encode(), C.byref(h)))
         else:

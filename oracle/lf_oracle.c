@@ -115,14 +115,27 @@ static void geom_eval(const orc_solver* s, double z, double* rin, double* rout, 
 
 /* Grid::Grid + Grid::Build, src/grid.cpp:3-42 (single rank: l_start = m_start = 0). */
 static void build_grid(orc_solver* s) {
+    const int table = s->p.geom_type == ORC_GEOM_ROWS;
     double a, b, c, d;
-    geom_eval(s, 0.0, &a, &b, &c, &d);
+    if (table) {
+        a = s->p.geom_rows[0][1];
+        b = s->p.geom_rows[1][1];
+    } else {
+        geom_eval(s, 0.0, &a, &b, &c, &d);
+    }
     s->r_0 = (a + b) / 2.0;
     for (int l = 0; l < s->rows; ++l) {
         const int l_global = l - 1;
         const double z = l_global * s->dz;
         double rin, rout, drin, drout;
-        geom_eval(s, z, &rin, &rout, &drin, &drout);
+        if (table) {
+            rin = s->p.geom_rows[0][l];
+            rout = s->p.geom_rows[1][l];
+            drin = s->p.geom_rows[2][l];
+            drout = s->p.geom_rows[3][l];
+        } else {
+            geom_eval(s, z, &rin, &rout, &drin, &drout);
+        }
         s->R[l] = rout - rin;
         s->dr[l] = s->R[l] / s->Mmax;
         for (int m = 0; m < s->cols; ++m) {

@@ -22,9 +22,10 @@ extern "C" {
 enum { ORC_NEUMANN = 0, ORC_DIRICHLET = 1, ORC_EXPRESSION = 2, ORC_AXIS_LF = 3 };
 
 /* Geometry kinds: the straight coaxial channel, the converging-diverging
- * channel r_in = a0 + a1 sin(pi z), r_out = b0 - b1 z^2, and the reference's
- * short_cathode_cosine (src/geometries/short_cathode_cosine.cpp:46-71). */
-enum { ORC_GEOM_STRAIGHT = 0, ORC_GEOM_CONDIV = 1, ORC_GEOM_CATHODE_COSINE = 2 };
+ * channel r_in = a0 + a1 sin(pi z), r_out = b0 - b1 z^2, the reference's
+ * short_cathode_cosine (src/geometries/short_cathode_cosine.cpp:46-71), and
+ * per-row wall data given by the caller (geom_rows). */
+enum { ORC_GEOM_STRAIGHT = 0, ORC_GEOM_CONDIV = 1, ORC_GEOM_CATHODE_COSINE = 2, ORC_GEOM_ROWS = 3 };
 
 /* One physical field's condition.  value: Neumann offset or Dirichlet value. */
 typedef struct {
@@ -62,6 +63,9 @@ typedef struct {
     /* Faces in the order l_lo, l_hi, m_lo, m_hi. */
     int nseg[4];
     const orc_seg* seg[4];
+    /* ORC_GEOM_ROWS: r_in, r_out, dr_in/dz, dr_out/dz at z = (l - 1) dz for
+     * l = 0 .. L_max + 1 (the points src/grid.cpp:20-42 evaluates). */
+    const double* geom_rows[4];
 } orc_params;
 
 typedef struct orc_solver orc_solver;

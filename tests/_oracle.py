@@ -15,7 +15,7 @@ ORACLE_SO = os.path.join(ORACLE_DIR, "liblf_oracle.so")
 REF_HARNESS = os.path.join(ORACLE_DIR, "_ref", "ref_harness")
 
 NEUMANN, DIRICHLET, EXPRESSION, AXIS_LF = 0, 1, 2, 3
-GEOM_STRAIGHT, GEOM_CONDIV, GEOM_CATHODE_COSINE = 0, 1, 2
+GEOM_STRAIGHT, GEOM_CONDIV, GEOM_CATHODE_COSINE, GEOM_ROWS = 0, 1, 2, 3
 
 # physical ids (include/fields.hpp member order)
 RHO, VZ, VR, VPHI, E, P, PT, HZ, HR, HPHI = range(10)
@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("ic_var_values", C.POINTER(C.c_double)),
         ("ic_noise_amp", C.c_double), ("ic_noise_seed", C.c_ulonglong),
         ("nseg", C.c_int * 4), ("seg", C.POINTER(Seg) * 4),
+        ("geom_rows", C.POINTER(C.c_double) * 4),
     ]
 
 
@@ -129,11 +130,19 @@ class Oracle:
         p.conv_threshold = case.get("conv_threshold", 0.0)
         p.check_frequency = case.get("check_frequency", 100)
         gt, gp = case.get("geom", (GEOM_STRAIGHT, [0.2, 0.8]))
-        p.geom_type = gt
-        for i, v in enumerate(gp):
-            p.geom_p[i] = v
-        ic = case.get("ic", {})
         self._keep = []
+        if case.get("geom_rows") is not None:
+            gt = GEOM_ROWS
+            rows = [np.ascontiguousarray(a, dtype=np.float64) for a in case["geom_rows"]]
+            assert all(a.shape == (case["L"] + 2,) for a in rows)
+            self._keep += rows
+            for i, a in enumerate(rows):
+                p.geom_rows[i] = a.ctypes.data_as(C.POINTER(C.c_double))
+        else:
+            for i, v in enumerate(gp):
+                p.geom_p[i] = v
+        p.geom_type = gt
+        ic = case.get("ic", {})
         for i, f in enumerate(BC_FIELDS):
             if f in ic:
                 b = str(ic[f]).encode()

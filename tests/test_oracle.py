@@ -241,3 +241,27 @@ def test_noise_generator_known_values():
     lib = O.lib()
     assert lib.orc_noise_u01(0, 0) == (0xE220A8397B1DCDAF >> 11) * 2.0 ** -53
     assert lib.orc_noise_u01(0, 1) == (0x6E789E6AA1B965F4 >> 11) * 2.0 ** -53
+
+
+def test_oracle_grid_from_rows():
+    """Per-row wall data (the caller's IGeometry evaluated at Grid::Build's
+    points): constant rows give the straight channel's grid bit for bit, and
+    a cubic profile gives r = (1 - frac) r_in + frac r_out, r_z likewise,
+    R = r_out - r_in, dr = R / M_max and r_0 from row 1 (z = 0), in
+    src/grid.cpp:20-42's operation order."""
+    L, M = 12, 7
+    straight = O.Oracle(dict(L=L, M=M, geom=(O.GEOM_STRAIGHT, [0.3, 0.9])))
+    const = [np.full(L + 2, 0.3), np.full(L + 2, 0.9), np.zeros(L + 2), np.zeros(L + 2)]
+    rows = O.Oracle(dict(L=L, M=M, geom_rows=const))
+    for a, b in ((rows.r, straight.r), (rows.r_z, straight.r_z), (rows.R, straight.R), (rows.dr, straight.dr)):
+        assert same_bits(a, b)
+    assert rows.r0 == straight.r0
+    z = (np.arange(L + 2) - 1.0) * (1.0 / L)
+    rin, rout = 0.2 + 0.05 * z ** 3, 0.8 - 0.03 * z
+    drin, drout = 0.15 * z ** 2, np.full(L + 2, -0.03)
+    o = O.Oracle(dict(L=L, M=M, geom_rows=[rin, rout, drin, drout]))
+    frac = (np.arange(M + 3) - 1.0) * (1.0 / M)
+    assert same_bits(o.r, (1.0 - frac)[None, :] * rin[:, None] + frac[None, :] * rout[:, None])
+    assert same_bits(o.r_z, (1.0 - frac)[None, :] * drin[:, None] + frac[None, :] * drout[:, None])
+    assert same_bits(o.R, rout - rin) and same_bits(o.dr, (rout - rin) / M)
+    assert o.r0 == (rin[1] + rout[1]) / 2.0
