@@ -371,8 +371,10 @@ def mode_sweep_converge(args):
 
 def mode_warmstart(args):
     import warmstart
+    levels = [float(v) for v in args.warm_levels.split(",")] if args.warm_levels else None
     warmstart.experiment(args.L, args.M, args.train, args.test, args.epochs, args.batch, args.lr, args.clip,
-                         args.plain_sample, args.seed, args.out)
+                         args.plain_sample, args.seed, args.out, warm_levels=levels,
+                         ref_threshold=args.ref_threshold, load=args.load)
 
 
 def mode_nn_train(args):
@@ -417,6 +419,10 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default="")
+    ap.add_argument("--warm-levels", default="", help="tighter warm-start stop levels, e.g. 1e-9,1e-10")
+    ap.add_argument("--ref-threshold", type=float, default=None,
+                    help="also build reference steady states converged to this level and report errors against them")
+    ap.add_argument("--load", default="", help="a saved warmstart_net.npz (skips generation and training)")
     ap.add_argument("--H", type=int, default=96)
     ap.add_argument("--W", type=int, default=40)
     args = ap.parse_args()

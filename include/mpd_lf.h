@@ -154,7 +154,10 @@ int mpd_write_vtk_fields(const char* yaml_text, const double* const* phys, const
  * per XCD, 2: four vertically adjacent tiles per workgroup marching in
  * alternate directions, chunk groups per XCD), "buffers" (1: raw buffer
  * addressing for states below 2^31 bytes, the default; 0: the plain-pointer
- * path larger states take), "steps_per_graph". */
+ * path larger states take), "steps_per_graph", "convergence_threshold" (a new
+ * level for the loop's convergence stop, for a config with a convergence
+ * check: a run that converged continues from its snapshot to the new level;
+ * a batch instance's level is taken at its batch's next run). */
 int mpd_set_option(mpd_solver* s, const char* key, double value);
 
 /* Benchmark helper: run `warmup` then `steps` loop steps with the current

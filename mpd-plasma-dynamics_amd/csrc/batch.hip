@@ -294,6 +294,8 @@ struct mpd_batch {
     // max_steps more steps each, <= 0: no cap besides its time.max_steps).
     void Run(int max_steps, mpd_run_result* out) {
         const int N = n();
+        // Per-instance convergence levels may have changed since the last run.
+        UploadArgs();
         std::vector<int> step0(N);
         std::vector<char> active(N, 0);
         // Every instance's loop state in one copy.

@@ -253,6 +253,15 @@ int mpd_set_option(mpd_solver* s, const char* key, double value) {
         }
         else if (k == "buffers") s->scfg.buffers = value != 0.0;
         else if (k == "steps_per_graph") s->steps_per_graph = (int)value;
+        else if (k == "convergence_threshold") {
+            // A new stop level for the loop's convergence check: a run that
+            // converged continues from its snapshot to the new level.
+            if (!(value > 0.0)) throw ArgError("convergence_threshold must be positive");
+            if (s->cfg.convergence_threshold <= 0.0 || !s->prev)
+                throw ArgError("convergence_threshold: the config has no convergence check");
+            s->cfg.convergence_threshold = value;
+            s->converged = false;
+        }
         else throw ArgError("unknown option '" + k + "'");
         for (auto& g : s->graph)
             if (g) {

@@ -252,18 +252,82 @@ def step_budget(train_steps, factor=1.5):
     return int(factor * train_steps.max()) if train_steps.size else None
 
 
+def set_threshold(b, n, level):
+    """A new convergence stop level for every instance of batch b."""
+    for i in range(n):
+        b.instance(i).set_option("convergence_threshold", level)
+
+
+def run_to_end(b, n, chunk, steps, finish=None, t_start=None, on_chunk=None):
+    """Runs batch b until every instance's loop has ended, in slices of
+    `chunk` steps; adds the steps taken to `steps` and records in `finish`
+    (seconds since t_start) when each instance ended.  on_chunk(res) may
+    restart instances (it returns the set it restarted).  Returns the last
+    results and the seconds spent in the runs."""
+    spent = 0.0
+    while True:
+        t0 = time.perf_counter()
+        res = b.run(chunk)
+        spent += time.perf_counter() - t0
+        steps += np.array([r["steps"] for r in res])
+        if finish is not None:
+            el = time.perf_counter() - t_start
+            for i, r in enumerate(res):
+                if r["done"] and np.isnan(finish[i]):
+                    finish[i] = el
+        if on_chunk is not None:
+            for i in on_chunk(res):
+                if finish is not None:
+                    finish[i] = np.nan
+        if all(r["done"] for r in res):
+            return res, spent
+
+
+def reference_states(cfgs, L, M, ref_threshold, max_steps, chunk=20000):
+    """Tightly converged FP64 steady states: each case's cold solve (bit-exact
+    with the oracle's) continued from its 1e-8 state to SolutionChange <
+    ref_threshold, or to max_steps more steps (stagnation at the rounding
+    floor).  Returns (fields, steps, reached)."""
+    n = len(cfgs)
+    b = mpd_amd.Batch(cfgs)
+    b.autotune(5)
+    steps = np.zeros(n, int)
+    run_to_end(b, n, chunk, steps)
+    set_threshold(b, n, ref_threshold)
+    cap = np.zeros(n, int)
+    while True:
+        res = b.run(chunk)
+        for i, r in enumerate(res):
+            steps[i] += r["steps"]
+            cap[i] += r["steps"]
+        if all(r["done"] or cap[i] >= max_steps for i, r in enumerate(res)):
+            break
+    reached = np.array([bool(r["converged"]) for r in res])
+    fields = steady_fields(b, n, L, M)
+    b.close()
+    return fields, steps, reached
+
+
 def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_change=1e-6, plain_sample=None,
-             chunk=1000, fallback=True, pred=None, budget=None):
+             chunk=1000, fallback=True, pred=None, budget=None, warm_levels=None, ref_threshold=None,
+             ref_max_steps=2000000):
     """Cold-started plain solves (each case alone, mpd_run) against the
     surrogate-warm-started batched sweep, both to the same threshold.  A
     warm case whose SolutionChange is still above stall_change after
     stall_steps steps is restarted cold (fallback), and so is one still
     running after `budget` steps (step_budget; None: no budget).
     plain_sample: time only that many plain solves and scale to all cases
-    (stated in the result)."""
+    (stated in the result).
+    warm_levels: tighter convergence levels the warm batch continues to after
+    `threshold` (the warm start's stop rule is the last one; the wall time and
+    steps reported are to it, and the states at every level are compared).
+    ref_threshold: also build tightly converged FP64 steady states
+    (reference_states) and report every state's per-field error against
+    them."""
     n = len(params)
     cfgs = [sweep.case_config(p, L, M, threshold=threshold, max_steps=2000000) for p in params]
     radii = grid_radii(params, L, M)
+    levels = [threshold] + [v for v in (warm_levels or []) if v < threshold]
 
     # Plain cold solves, one case at a time (the reference's way: one run per case).
     m = n if plain_sample is None else min(n, plain_sample)
@@ -304,11 +368,11 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     mid = np.array([r["change"] for r in res])
     res = b.run(stall_steps - stall_steps // 2)
     steps += np.array([r["steps"] for r in res])
+    relax = time.perf_counter() - t1
     finish = np.full(n, np.nan)
-    el = time.perf_counter() - t1
     for i, r in enumerate(res):
         if r["done"]:
-            finish[i] = el
+            finish[i] = relax
     stall = [i for i, r in enumerate(res) if fallback and i not in rejected and
              stalled(r["done"], r["converged"], r["change"], mid[i], stall_change)]
     for i in stall:
@@ -316,24 +380,34 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
         finish[i] = np.nan
     restarted = set(stall) | set(rejected)
     late = []
-    while True:
-        res = b.run(chunk)
-        el = time.perf_counter() - t1
-        steps += np.array([r["steps"] for r in res])
-        for i, r in enumerate(res):
-            if r["done"] and np.isnan(finish[i]):
-                finish[i] = el
+
+    def over_budget(res):
         # Step budget: a warm case still running past it starts again cold.
+        out = []
         if fallback and budget is not None:
             for i, r in enumerate(res):
                 if not r["done"] and i not in restarted and steps[i] > budget:
                     b.instance(i).cold_restart(cfgs[i])
                     restarted.add(i)
                     late.append(i)
-        if all(r["done"] for r in res):
-            break
-    warm_wall = time.perf_counter() - t1
-    warm_fields = steady_fields(b, n, L, M)
+                    out.append(i)
+        return out
+
+    level_fields, level_steps, level_wall = [], [], []
+    res, spent = run_to_end(b, n, chunk, steps, finish, t1, over_budget)
+    relax += spent
+    for k, level in enumerate(levels):
+        if k > 0:
+            # Per-case finish times are those of the last level.
+            set_threshold(b, n, level)
+            finish[:] = np.nan
+            res, spent = run_to_end(b, n, chunk, steps, finish, t1)
+            relax += spent
+        level_steps.append(steps.copy())
+        level_wall.append(relax)
+        level_fields.append(steady_fields(b, n, L, M))
+    warm_wall = relax
+    warm_fields = level_fields[-1]
     b.close()
     progress(f"warm batch: inference {infer:.3f}s, relax {warm_wall:.2f}s, {len(rejected)} rejected, "
              f"{len(stall)} stalled, {len(late)} over the step budget")
@@ -341,11 +415,12 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     errs = np.array([field_errors(warm_fields[i], cold_fields[i]) for i in range(n)])
     pred_errs = np.array([field_errors(pred[i], cold_fields[i]) for i in range(n)])
     cold_steps = np.array([r["steps"] for r in cold_res])
-    return {
-        "cases": n, "grid": [L, M], "threshold": threshold,
+    out = {
+        "cases": n, "grid": [L, M], "threshold": threshold, "warm_threshold": levels[-1],
         "plain_cold_total_s": plain_total, "plain_timed_cases": m,
         "plain_cold_steps_mean": float(np.mean(plain_steps)),
-        "cold_batch_s": cold_wall, "cold_steps_p50": float(np.median(cold_steps)),
+        "cold_batch_s": cold_wall, "cold_steps_total": int(cold_steps.sum()),
+        "cold_steps_p50": float(np.median(cold_steps)),
         "cold_steps_max": int(cold_steps.max()),
         "warm_inference_s": infer, "warm_batch_s": warm_wall, "warm_total_s": infer + warm_wall,
         "warm_steps_p50": float(np.median(steps)), "warm_steps_max": int(steps.max()),
@@ -363,44 +438,92 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
         "max_field_err_warm_vs_cold": dict(zip(SG.OUT_FIELDS, errs.max(axis=0).tolist())),
         "max_field_err_prediction_vs_cold": dict(zip(SG.OUT_FIELDS, pred_errs.max(axis=0).tolist())),
         "median_field_err_prediction_vs_cold": dict(zip(SG.OUT_FIELDS, np.median(pred_errs, axis=0).tolist())),
-    }, warm_fields, cold_fields
+        "warm_levels": [{"threshold": lv, "steps_total": int(level_steps[k].sum()),
+                         "steps_p50": float(np.median(level_steps[k])), "relax_s": level_wall[k]}
+                        for k, lv in enumerate(levels)],
+    }
+    if ref_threshold is not None:
+        t2 = time.perf_counter()
+        ref_fields, ref_steps, reached = reference_states(cfgs, L, M, ref_threshold, ref_max_steps)
+        progress(f"reference states: {reached.sum()}/{n} reached {ref_threshold:g} "
+                 f"({time.perf_counter() - t2:.1f}s, steps p50 {np.median(ref_steps):.0f})")
+
+        def worst(states):
+            e = np.array([field_errors(states[i], ref_fields[i]) for i in range(n)])
+            return dict(zip(SG.OUT_FIELDS, e.max(axis=0).tolist())), float(e.max())
+
+        cold_ref, cold_max = worst(cold_fields)
+        out.update({"reference": {"threshold": ref_threshold, "reached": int(reached.sum()),
+                                  "steps_p50": float(np.median(ref_steps)), "steps_max": int(ref_steps.max()),
+                                  "max_steps_beyond_threshold": ref_max_steps},
+                    "max_field_err_cold_vs_reference": cold_ref, "max_err_cold_vs_reference": cold_max})
+        for k, lv in enumerate(levels):
+            e, mx = worst(level_fields[k])
+            out["warm_levels"][k].update({"max_field_err_vs_reference": e, "max_err_vs_reference": mx})
+        out["max_field_err_warm_vs_reference"] = out["warm_levels"][-1]["max_field_err_vs_reference"]
+        out["max_err_warm_vs_reference"] = out["warm_levels"][-1]["max_err_vs_reference"]
+    return out, warm_fields, cold_fields
+
+
+def load_network(path, L, M, seed=2024):
+    """A network and normalizer saved by experiment(out=...) (its
+    warmstart_net.npz), with the training cases' cold step counts."""
+    z = np.load(path)
+    net = SG.UNet(L, M + 1, seed=seed)
+    nl = len(net.layers)
+    net.load_params([z[f"w{k}"] for k in range(nl)], [z[f"b{k}"] for k in range(nl)])
+    norm = Normalizer(None, mean=z["mean"], std=z["std"])
+    return net, norm, z["train_steps"] if "train_steps" in z else np.array([])
 
 
 def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, clip=None, plain_sample=16,
-               seed=2024, out=""):
+               seed=2024, out="", warm_levels=None, ref_threshold=None, load=""):
     """The whole experiment: train_n steady states generated, the network
-    trained on them, test_n held-out cases evaluated (evaluate); prints the
-    result as JSON and, with `out`, writes it and the trained weights there."""
-    t0 = time.perf_counter()
-    params, fields, steps, conv = generate(train_n, L, M, seed)
-    gen_s = time.perf_counter() - t0
-    keep = conv
-    progress(f"generated {train_n} steady states in {gen_s:.1f}s ({keep.sum()} converged, steps p50 "
-             f"{np.median(steps):.0f} max {steps.max()})")
-    params, fields = params[keep], fields[keep]
-    radii = grid_radii(params, L, M)
-    base = SG.baseline_fields(params, radii, L, M, GAMMA)
-    norm = Normalizer(fields - base)
-    X = SG.features(params, radii, L, M)
-    T = norm.encode(fields - base).astype(np.float32)
-    net = SG.UNet(L, M + 1, seed=seed)
-    progress(f"network: {net.n_params()} parameters, {len(net.layers)} convs")
-    t1 = time.perf_counter()
-    tstats = {}
-    losses = train(net, X, T, epochs, batch, lr, seed=seed, clip=clip, stats=tstats)
-    train_s = time.perf_counter() - t1
+    trained on them (or a saved one loaded), test_n held-out cases evaluated
+    (evaluate); prints the result as JSON and, with `out`, writes it and the
+    trained weights there."""
+    if load:
+        net, norm, train_steps = load_network(load, L, M, seed)
+        result_extra = {"loaded_network": os.path.basename(load)}
+        progress(f"network loaded from {load}")
+    else:
+        t0 = time.perf_counter()
+        params, fields, steps, conv = generate(train_n, L, M, seed)
+        gen_s = time.perf_counter() - t0
+        keep = conv
+        progress(f"generated {train_n} steady states in {gen_s:.1f}s ({keep.sum()} converged, steps p50 "
+                 f"{np.median(steps):.0f} max {steps.max()})")
+        params, fields = params[keep], fields[keep]
+        train_steps = steps[keep]
+        radii = grid_radii(params, L, M)
+        base = SG.baseline_fields(params, radii, L, M, GAMMA)
+        norm = Normalizer(fields - base)
+        X = SG.features(params, radii, L, M)
+        T = norm.encode(fields - base).astype(np.float32)
+        net = SG.UNet(L, M + 1, seed=seed)
+        progress(f"network: {net.n_params()} parameters, {len(net.layers)} convs")
+        t1 = time.perf_counter()
+        tstats = {}
+        losses = train(net, X, T, epochs, batch, lr, seed=seed, clip=clip, stats=tstats)
+        train_s = time.perf_counter() - t1
+        result_extra = {"train_cases": int(keep.sum()), "generate_s": gen_s, "train_s": train_s, "epochs": epochs,
+                        "clip": clip, "rollbacks": tstats["rollbacks"], "final_loss": losses[-1],
+                        "first_loss": losses[0], "train_steps_p50": float(np.median(train_steps)),
+                        "train_steps_max": int(train_steps.max())}
+        if out:
+            os.makedirs(out, exist_ok=True)
+            sd = net.state_dict()
+            np.savez(os.path.join(out, "warmstart_net.npz"), **{f"w{k}": w for k, w in enumerate(sd["w"])},
+                     **{f"b{k}": b for k, b in enumerate(sd["b"])}, mean=norm.mean, std=norm.std,
+                     train_steps=train_steps)
     test_params = sweep.sample_params(test_n, seed + 1)
     result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=plain_sample,
-                            budget=step_budget(steps[keep]))
-    result.update({"train_cases": int(keep.sum()), "generate_s": gen_s, "train_s": train_s, "epochs": epochs,
-                   "clip": clip, "rollbacks": tstats["rollbacks"],
-                   "final_loss": losses[-1], "first_loss": losses[0], "n_params": net.n_params()})
+                            budget=step_budget(train_steps), warm_levels=warm_levels, ref_threshold=ref_threshold)
+    result.update(result_extra)
+    result["n_params"] = net.n_params()
     print(json.dumps(result, indent=1), flush=True)
     if out:
         os.makedirs(out, exist_ok=True)
         with open(os.path.join(out, "warmstart_result.json"), "w") as f:
             json.dump(result, f, indent=1)
-        sd = net.state_dict()
-        np.savez(os.path.join(out, "warmstart_net.npz"), **{f"w{k}": w for k, w in enumerate(sd["w"])},
-                 **{f"b{k}": b for k, b in enumerate(sd["b"])}, mean=norm.mean, std=norm.std)
     return result

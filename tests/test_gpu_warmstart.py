@@ -5,10 +5,12 @@ on a small grid: 48 x 24 (48 x 25 cells; the 4-level network pads them to
 * The cold steady state the sweeps are built on is the oracle's, bit for bit
   (the step count at which the residual falls below 1e-8 included).
 * A network trained on the GPU (the full 4-level, 32-channel architecture)
-  from solver-generated steady states gives warm starts that reach the same
-  threshold in fewer steps than cold starts -- the gain over the method
-  without it -- and converge to the cold steady state within 1e-5 relative
-  per field (both are states of residual below 1e-8, not the same state).
+  from solver-generated steady states gives warm starts that take fewer
+  steps than cold starts -- the gain over the method without it.  The warm
+  start's stop rule relaxes to SolutionChange 1e-9 (one decade below the
+  sweep threshold): its per-field error against the FP64 steady state (the
+  cold solve continued to 1e-12, bit-exact with the oracle's) is at most
+  1e-6 relative, and it still takes fewer steps than the cold solve to 1e-8.
 * Junk predictions (negative density, a NaN) are not used: those cases
   start cold, and reproduce the cold steady state bit for bit; an
   admissible but poor prediction is still relaxed to the same steady state
@@ -72,14 +74,19 @@ def test_trained_warm_start_saves_steps(ws, trained):
     import sweep
     net, norm = trained
     params = sweep.sample_params(8, 77)
-    r, warm, cold = ws.evaluate(net, norm, params, L, M, plain_sample=2)
+    r, warm, cold = ws.evaluate(net, norm, params, L, M, plain_sample=2, warm_levels=[1e-9], ref_threshold=1e-12)
     print({k: r[k] for k in ("cold_steps_p50", "warm_steps_p50", "steps_ratio_cold_over_warm", "fallbacks",
                              "speedup_vs_cold_batch", "speedup_vs_plain_cold")}, flush=True)
-    print("warm vs cold:", r["max_field_err_warm_vs_cold"], flush=True)
-    assert r["all_converged_cold"] and r["all_converged_warm"]
+    print("levels:", r["warm_levels"], flush=True)
+    print("cold (1e-8) vs reference:", r["max_field_err_cold_vs_reference"], flush=True)
+    print("warm (1e-9) vs reference:", r["max_field_err_warm_vs_reference"], flush=True)
+    assert r["all_converged_cold"] and r["all_converged_warm"] and r["reference"]["reached"] == len(params)
     assert r["fallbacks"] <= 1
-    assert r["steps_ratio_cold_over_warm"] > 1.2
-    assert max(r["max_field_err_warm_vs_cold"].values()) < 1e-5
+    # To the same level the warm start saves steps; relaxing a decade
+    # further it still takes fewer than the cold solve to 1e-8.
+    assert r["cold_steps_total"] > 1.2 * r["warm_levels"][0]["steps_total"]
+    assert r["cold_steps_total"] > r["warm_levels"][-1]["steps_total"]
+    assert r["max_err_warm_vs_reference"] <= 1e-6
 
 
 def test_junk_warm_start_falls_back_to_cold(ws, trained):

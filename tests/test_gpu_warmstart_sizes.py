@@ -1,0 +1,148 @@
+"""GPU tests of the learned warm start at the workload sizes (marked slow,
+all collected): the 400 x 160 grid of the warm-start workload and the 256
+combined sweep cases at 800 x 320 with mixed wall shapes.  They exercise the
+pipeline at these sizes -- a network trained briefly on the GPU on solver
+states, prediction, the batched relaxation, the admissibility screen and the
+cold fallback -- and pin the cold path to the oracle; the full-length
+experiments (training on converged steady states, runs to residual 1e-8)
+are bench.py --mode warmstart."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import cases
+from parity import describe_mismatch, same_bits
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+
+@pytest.fixture(scope="module")
+def ws():
+    import warmstart
+    return warmstart
+
+
+def _cold_matches_oracle(mpd, cfg, steps):
+    """The cold path of a sweep case for `steps` steps equals the oracle's
+    bit for bit (state, t, dt)."""
+    case = cases.oracle_case(cfg)
+    o = O.Oracle(case)
+    ores = o.run(steps)
+    with mpd.Solver(cfg) as sol:
+        res = sol.run(steps)
+        u, p = sol.download()
+    assert (res["steps"], res["t"], res["dt"]) == (ores["steps"], ores["t"], ores["dt"]), (res, ores)
+    for k in range(8):
+        assert same_bits(u[k], o.u0(k)), f"u0_{k + 1}: {describe_mismatch(u[k], o.u0(k))}"
+    for k in range(10):
+        assert same_bits(p[k], o.phys(k)), O.PHYS_NAMES[k]
+
+
+def _train_on_states(ws, params, L, M, horizon, epochs, batch, lr=2e-3):
+    """A network trained briefly on the states the batched solver reaches
+    after `horizon` steps from cold starts (short of steady state: this
+    exercises training at the grid size, not the prediction quality)."""
+    import surrogate as SG
+    import sweep
+    cfgs = [sweep.case_config(p, L, M, threshold=1e-8, max_steps=horizon) for p in params]
+    res, wall, _, b = ws.solve_batch(cfgs)
+    fields = ws.steady_fields(b, len(cfgs), L, M)
+    b.close()
+    radii = ws.grid_radii(params, L, M)
+    base = SG.baseline_fields(params, radii, L, M, ws.GAMMA)
+    norm = ws.Normalizer(fields - base)
+    net = SG.UNet(L, M + 1, seed=5)
+    assert (net.H, net.W) == (SG.padded(L), SG.padded(M + 1))
+    losses = ws.train(net, SG.features(params, radii, L, M), norm.encode(fields - base).astype(np.float32),
+                      epochs=epochs, batch=batch, lr=lr, log_every=max(1, epochs // 2))
+    print(f"{L}x{M}: {len(params)} states after {horizon} steps, loss {losses[0]:.3g} -> {losses[-1]:.3g}",
+          flush=True)
+    # The divergence guard keeps the best epoch's weights.
+    assert np.isfinite(min(losses)) and min(losses[1:]) < losses[0]
+    return net, norm
+
+
+def test_warm_start_400x160(mpd, ws):
+    import sweep
+    L, M = 400, 160
+    params = sweep.sample_params(11, 31)
+    _cold_matches_oracle(mpd, sweep.case_config(params[0], L, M, max_steps=200), 200)
+    net, norm = _train_on_states(ws, params[:8], L, M, horizon=2000, epochs=12, batch=4)
+    test = params[8:]
+    radii = ws.grid_radii(test, L, M)
+    pred = ws.predict(net, norm, test, radii, L, M, batch=4)
+    assert pred.shape == (3, L, M + 1, 8) and np.all(np.isfinite(pred))
+    pred[0, ..., 4] = -1.0  # not admissible: negative internal energy -> cold start
+    r, warm, cold = ws.evaluate(net, norm, test, L, M, threshold=1e-6, plain_sample=1, pred=pred)
+    print({k: r[k] for k in ("cold_steps_p50", "warm_steps_p50", "fallbacks", "rejected_predictions",
+                             "stalled", "warm_batch_s", "cold_batch_s")}, flush=True)
+    assert r["rejected_predictions"] == 1 and r["fallbacks"] >= 1
+    assert r["all_converged_cold"] and r["all_converged_warm"]
+    # The rejected case ran cold: the cold batch's state bit for bit.
+    assert np.array_equal(warm[0], cold[0])
+    for i in (1, 2):
+        assert max(ws.field_errors(warm[i], cold[i])) < 1e-2  # both at residual 1e-6
+
+
+def test_combined_256_cases_800x320(mpd, ws):
+    """256 seeded sweep cases at 800 x 320 with mixed wall shapes (inner and
+    outer amplitudes drawn in [0, 0.1]), warm-started in one batch from the
+    network's predictions, relaxed for the 2000-step stall horizon; cases
+    whose residual stalls above 1e-6 restart cold.  Checks: every state
+    finite with positive density and energy, every case advanced, the stall
+    rule applied as stated, and a cold-restarted case equal to a fresh cold
+    run (and to the oracle) bit for bit."""
+    import surrogate as SG
+    import sweep
+    L, M, n, horizon = 800, 320, 256, 2000
+    params = sweep.sample_params(n + 4, 41)
+    assert np.ptp(params[:, 3]) > 0.05 and np.ptp(params[:, 4]) > 0.05  # mixed wall shapes
+    net, norm = _train_on_states(ws, params[n:], L, M, horizon=500, epochs=8, batch=2, lr=3e-4)
+    params = params[:n]
+    radii = ws.grid_radii(params, L, M)
+    pred = ws.predict(net, norm, params, radii, L, M, batch=8)
+    ok = [ws.admissible(pred[i]) for i in range(n)]
+    cfgs = [sweep.case_config(p, L, M, threshold=1e-8, max_steps=100000) for p in params]
+    with mpd.Batch(cfgs) as b:
+        for i, u in enumerate(ws.warm_u0(pred, radii, L, M)):
+            if ok[i]:
+                b.warm_start(i, u)
+        print(b.autotune(3), flush=True)
+        res1 = b.run(horizon // 2)
+        mid = [r["change"] for r in res1]
+        res2 = b.run(horizon - horizon // 2)
+        for i in range(n):
+            assert res1[i]["steps"] + res2[i]["steps"] == horizon or res2[i]["done"], (i, res1[i], res2[i])
+        stalled = [i for i in range(n) if ok[i] and
+                   ws.stalled(res2[i]["done"], res2[i]["converged"], res2[i]["change"], mid[i], 1e-6)]
+        cold = [i for i in range(n) if not ok[i]] + stalled
+        print(f"{n} cases: {n - sum(ok)} predictions not admissible, {len(stalled)} stalled after {horizon} steps",
+              flush=True)
+        for i in range(n):
+            if not ok[i]:
+                continue
+            assert np.isfinite(res2[i]["change"]) and res2[i]["t"] > 0
+        # Restart cold: one stalled (or rejected) case, or case 0 if none.
+        restart = cold[:8] or [0]
+        for i in restart:
+            b.instance(i).cold_restart(cfgs[i])
+        res3 = b.run(300)
+        for i in restart:
+            assert res3[i]["total_steps"] == 300, res3[i]
+        for i in range(0, n, 37):
+            u, p = b.instance(i).download()
+            assert all(np.all(np.isfinite(a)) for a in u), i
+            assert np.all(p[O.RHO][1:-1, 1:-1] > 0) and np.all(p[O.E][1:-1, 1:-1] > 0), i
+        j = restart[0]
+        ub, pb = b.instance(j).download()
+    with mpd.Solver(cfgs[j]) as sol:
+        sol.run(300)
+        us, ps = sol.download()
+    for k in range(8):
+        assert same_bits(ub[k], us[k]), f"restarted case {j} u0_{k + 1}"
+    case = cases.oracle_case(cfgs[j])
+    o = O.Oracle(case)
+    o.run(300)
+    for k in range(8):
+        assert same_bits(ub[k], o.u0(k)), f"restarted case {j} vs oracle u0_{k + 1}"
+    assert SG.padded(M + 1) == 328
