@@ -39,6 +39,10 @@ typedef struct {
     int converged;   /* SolutionChange fell below convergence.threshold */
     int done;        /* the time loop has ended: t >= T, converged, or time.max_steps reached
                         (0 when only this call's max_steps stopped it) */
+    int fast_div;    /* 1: the solver still uses the split division (gpu.fast_div); 0: IEEE
+                        division, by config or since a state left the split division's exact
+                        range (|u_1| in [2^-30, 2^31), other conserved values 0 or in
+                        [2^-100, 2^61), radii in [2^-8, 2^8]); results are IEEE-exact either way */
 } mpd_run_result;
 
 /* Build a solver from a config.yaml path / text: SimConfig::Load

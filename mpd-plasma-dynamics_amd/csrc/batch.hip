@@ -290,12 +290,48 @@ struct mpd_batch {
         HIPCHK(hipStreamSynchronize(stream));
     }
 
+    void SwitchToIeeeDivision(int max_steps, const std::vector<int>& step0, std::vector<char>& active) {
+        const int N = n();
+        fast_div = false;
+        for (int i = 0; i < N; ++i) {
+            mpd_solver& s = *inst[i];
+            if (s.scfg.fast_div) {
+                s.scfg.fast_div = false;
+                s.fast_div_step = s.step;
+            }
+            const int taken = s.step - step0[i];
+            const bool capped = max_steps > 0 && taken >= max_steps;
+            h_st.p[i] = s.LoopSeed(capped ? 0 : mpd_solver::Remaining(max_steps, taken));
+            if (capped) h_st.p[i].done = 1;
+            active[i] = (active[i] && !h_st.p[i].done) ? 1 : 0;
+        }
+        HIPCHK(hipMemcpyAsync(st, h_st.p, sizeof(MpdLoopState) * N, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (graph) {
+            HIPCHK(hipGraphExecDestroy(graph));
+            graph = nullptr;
+        }
+    }
+
     // The batched time loop: every instance runs its own loop (at most
     // max_steps more steps each, <= 0: no cap besides its time.max_steps).
     void Run(int max_steps, mpd_run_result* out) {
         const int N = n();
         // Per-instance convergence levels may have changed since the last run.
         UploadArgs();
+        // An instance whose uploaded state (a warm start) is outside the split
+        // division's range has switched itself to IEEE division: the batch's
+        // launches cover every instance, so the batch follows.
+        bool all_fast = true;
+        for (int i = 0; i < N; ++i) all_fast = all_fast && inst[i]->scfg.fast_div;
+        if (fast_div && !all_fast) {
+            fast_div = false;
+            for (int i = 0; i < N; ++i) inst[i]->scfg.fast_div = false;
+            if (graph) {
+                HIPCHK(hipGraphExecDestroy(graph));
+                graph = nullptr;
+            }
+        }
         std::vector<int> step0(N);
         std::vector<char> active(N, 0);
         // Every instance's loop state in one copy.
@@ -316,11 +352,14 @@ struct mpd_batch {
         }
         if (any1) {
             ReadLoops();
+            bool range = false;
             for (int i = 0; i < N; ++i) {
                 if (!active[i] || !inst[i]->step1) continue;
                 inst[i]->Absorb(h_st.p[i]);
                 if (h_st.p[i].done) active[i] = 0;
+                range = range || h_st.p[i].range_stop != 0;
             }
+            if (range) SwitchToIeeeDivision(max_steps, step0, active);
         }
         for (;;) {
             bool any = false;
@@ -358,6 +397,12 @@ struct mpd_batch {
                     if (h_st.p[i].done) active[i] = 0;
                 }
             }
+            // A new state outside the split division's range (its instance
+            // idles behind range_stop): the batch continues with IEEE
+            // division, every loop re-seeded from its host mirror.
+            bool range = false;
+            for (int i = 0; i < N; ++i) range = range || h_st.p[i].range_stop != 0;
+            if (range) SwitchToIeeeDivision(max_steps, step0, active);
         }
         if (out) {
             for (int i = 0; i < N; ++i) {
@@ -371,6 +416,7 @@ struct mpd_batch {
                 r.converged = s.converged ? 1 : 0;
                 r.done = (!(s.t < s.cfg.T) || s.converged || (s.cfg.max_steps > 0 && s.step >= s.cfg.max_steps)) ? 1
                                                                                                                : 0;
+                r.fast_div = fast_div ? 1 : 0;
             }
         }
     }

@@ -35,6 +35,7 @@ struct StepArgs {
     int bc_area;          // work-area slots per lane of the boundary kernel (bc_area_slots)
     MpdLoopState* st;
     const uint64_t* bc;   // the BC program's image (BcImage layout, bc_words words)
+    const double* ringgeo;  // per boundary cell geometry (RingGeo layout, ring_compact_index order)
 };
 
 // One workgroup's entry in a batched launch's work list: its instance and,
@@ -130,6 +131,26 @@ __host__ __device__ inline bool ring_cell_compact(int idx, int L, int Mn, int& l
     else { l = idx - (2 * Mn + L - 2) + 2; m = Mn; }
     return idx < ring_lanes_compact(L, Mn);
 }
+
+// Index of boundary cell (l, m) in the compact order (the inverse of
+// ring_cell_compact).
+__host__ __device__ inline int ring_compact_index(int l, int m, int L, int Mn) {
+    if (l == 1) return m - 1;
+    if (l == L) return Mn + m - 1;
+    if (m == 1) return 2 * Mn + l - 2;
+    return 2 * Mn + (L - 2) + l - 2;
+}
+
+// Geometry a boundary cell's face chains read, precomputed on the host per
+// boundary cell (so the boundary kernel loads it with the cell states, in
+// one round trip, instead of per-row wall data in dependent loads): r and
+// r_z of the cell, of its row neighbour (l +- 1 inwards; the cell itself on
+// a column), of its column neighbour (m +- 1 inwards) and of the diagonal
+// between them, dr and z of the cell's row and of the row neighbour's.
+// Stored field-major: entry e of cell c at ringgeo[e * n + c], n =
+// ring_lanes_compact(L, Mn).
+enum { RG_R = 0, RG_RZ, RG_DR, RG_Z, RG_R_ROW, RG_RZ_ROW, RG_DR_ROW, RG_Z_ROW, RG_R_COL, RG_RZ_COL, RG_R_DIAG,
+       RG_RZ_DIAG, RG_N };
 
 __host__ __device__ inline int ring_lanes(int L, int Mn) {
     const int g1 = ((Mn - 2) + 63) / 64 * 64, gc = ((L - 2) + 63) / 64 * 64;
@@ -278,6 +299,28 @@ __device__ __forceinline__ void rec_phys_shared(Rec& c, double gm1) {
     const double e = div_by(c.u[4], c.u[0], yu);
     c.p = gm1 * c.rho * e;
     c.P = c.p + 0.5 * (c.Hz * c.Hz + c.Hr * c.Hr + c.Hphi * c.Hphi);
+}
+
+// Whether the next step's split divisions on conserved values u are
+// bit-identical to IEEE division: |u_1| in [2^-30, 2^31) and every other
+// value zero or |u_k| in [2^-100, 2^61) (binary exponents from
+// v_frexp_exp_i32_f64, which gives 0 for a zero).  With every radius in
+// [2^-8, 2^8] (checked on the host) the physical values, their flux
+// products and every difference the step divides lie between 2^-331 and
+// 2^221 (or are zero), inside the range div_by states.  A step whose new
+// state fails the test flags the loop state; the host continues with IEEE
+// division from the next step on.
+__device__ __forceinline__ bool split_div_safe(const double* u) {
+    bool ok = (unsigned)(__builtin_amdgcn_frexp_exp(u[0]) + 29) <= 60u;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) ok = ok && (unsigned)(__builtin_amdgcn_frexp_exp(u[k]) + 99) <= 160u;
+    return ok;
+}
+
+// Flags the loop state when any lane of the wave saw a value outside the
+// range (one atomic per wave, and only then).
+__device__ __forceinline__ void flag_range(MpdLoopState* st, bool bad) {
+    if (__builtin_amdgcn_ballot_w64(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(&st->range_bits, 1u);
 }
 
 struct DivExact {

@@ -136,7 +136,7 @@ __device__ __forceinline__ void load_rec(const StepArgs& a, int l, int m, Rec& c
 // step waits for its exact wave-speed rescan (fallback): the host runs the
 // rescan, which finishes that step, and launches the remaining steps again.
 __device__ __forceinline__ bool loop_idle(const StepArgs& a) {
-    return (a.st->done | a.st->fallback) != 0;
+    return (a.st->done | a.st->fallback | a.st->range_stop) != 0;
 }
 
 __device__ __forceinline__ double step_dt(const StepArgs& a, double dt_explicit, int loop) {
@@ -153,6 +153,7 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
     const int l = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
     const double dt = step_dt(a, dt_explicit, loop);
     uint64_t key = 0;
+    bool rbad = false;
     if (m <= a.Mn - 1 && l <= a.L) {
         Rec N, S, E, W, C;
         load_rec<kStep1>(a, l + 1, m, N);
@@ -172,11 +173,13 @@ __global__ void __launch_bounds__(256) lf_stencil_simple(StepArgs a, double dt_e
             for (int k = 0; k < 8; ++k) a.u_out[(int64_t)k * a.fstride + i] = o[k];
             if (kSpeed) key = speed_key(wave_speed_u(o, C.r, a.gamma, a.gm1));
         }
+        rbad = !split_div_safe(o);
     }
     if (kSpeed) {
         key = wave_max_u64(key);
         if ((threadIdx.x & 63) == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
     }
+    flag_range(a.st, rbad);
 }
 
 // A per-row geometry value (row l wave-uniform) read through the constant
@@ -365,7 +368,7 @@ struct TileCtx {
 // arithmetic is the same either way.
 template <bool kSpeed, bool kBuf, class Div, bool kUp>
 __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, const StateIO<kBuf>& io, int l,
-                                          Rec& S, const Rec& C, Rec& N, uint64_t& key) {
+                                          Rec& S, const Rec& C, Rec& N, uint64_t& key, bool& rbad) {
     const int lp = kUp ? l - 2 : l + 2;
     const bool pre = kUp ? (lp >= x.l0 - 1) : (lp <= x.l1);
     double raw[8];
@@ -395,6 +398,8 @@ __device__ __forceinline__ void march_row(const StepArgs& a, const TileCtx& x, c
             }
         }
     }
+    // Rows 1 and L are screened by the boundary kernel, which writes them.
+    if (x.outlane && !rb) rbad = rbad || !split_div_safe(o);
     if (pre) row_rec(a, lp, x.m, raw, kUp ? N : S);
 }
 
@@ -437,6 +442,7 @@ __device__ __forceinline__ void stencil_tile(const StepArgs& a, double dt_explic
     io.init(a, (uint32_t)phys_col(min(max(x.m, 0), a.Mn + 1)) * 8u, x.st_off);
     Rec A, B, C;
     uint64_t key = 0;
+    bool rbad = false;
     if (!up) {
         {
             double raw[8];
@@ -449,12 +455,12 @@ __device__ __forceinline__ void stencil_tile(const StepArgs& a, double dt_explic
         }
         int l = l0;
         for (; l + 3 <= x.l1; l += 3) {
-            march_row<kSpeed, kBuf, Div, false>(a, x, io, l, A, B, C, key);
-            march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 1, B, C, A, key);
-            march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 2, C, A, B, key);
+            march_row<kSpeed, kBuf, Div, false>(a, x, io, l, A, B, C, key, rbad);
+            march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 1, B, C, A, key, rbad);
+            march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 2, C, A, B, key, rbad);
         }
-        if (l < x.l1) march_row<kSpeed, kBuf, Div, false>(a, x, io, l, A, B, C, key);
-        if (l + 1 < x.l1) march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 1, B, C, A, key);
+        if (l < x.l1) march_row<kSpeed, kBuf, Div, false>(a, x, io, l, A, B, C, key, rbad);
+        if (l + 1 < x.l1) march_row<kSpeed, kBuf, Div, false>(a, x, io, l + 1, B, C, A, key, rbad);
     } else {
         // Rows l1-1 down to l0: A, B, C start as rows l1-2, l1-1, l1 (all
         // within 0 .. L+1 since l0 >= 1 and l1 <= L+1).
@@ -470,17 +476,18 @@ __device__ __forceinline__ void stencil_tile(const StepArgs& a, double dt_explic
         }
         int l = l1 - 1;
         for (; l - 2 >= l0; l -= 3) {
-            march_row<kSpeed, kBuf, Div, true>(a, x, io, l, A, B, C, key);
-            march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 1, C, A, B, key);
-            march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 2, B, C, A, key);
+            march_row<kSpeed, kBuf, Div, true>(a, x, io, l, A, B, C, key, rbad);
+            march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 1, C, A, B, key, rbad);
+            march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 2, B, C, A, key, rbad);
         }
-        if (l >= l0) march_row<kSpeed, kBuf, Div, true>(a, x, io, l, A, B, C, key);
-        if (l - 1 >= l0) march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 1, C, A, B, key);
+        if (l >= l0) march_row<kSpeed, kBuf, Div, true>(a, x, io, l, A, B, C, key, rbad);
+        if (l - 1 >= l0) march_row<kSpeed, kBuf, Div, true>(a, x, io, l - 1, C, A, B, key, rbad);
     }
     if (kSpeed) {
         key = wave_max_u64(key);
         if (lane == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
     }
+    flag_range(a.st, rbad);
 }
 
 template <bool kSpeed, bool kBuf, class Div>
@@ -698,22 +705,45 @@ __device__ __forceinline__ void face_nb(const StepArgs& a, int f, int l, int m, 
     else mn = a.Mn - 1;
 }
 
+// Geometry symbols of one face-chain evaluation at a cell: its row's dr,
+// its r, r_z and z, and its face neighbour's r and r_z.
+struct ChainGeo {
+    double dr, r, rz, z, rnb, rznb;
+};
+
+// The same read from the per-row wall data (the AxisLF stencil's l +- 1
+// chains, outside the boundary geometry table's cells).
+__device__ __forceinline__ ChainGeo chain_geo_rows(const StepArgs& a, int f, int l, int m) {
+    int ln, mn;
+    face_nb(a, f, l, m, ln, mn);
+    return ChainGeo{a.dr[l], cell_r(a, l, m), cell_rz(a, l, m), (double)(l - 1) * a.dz, cell_r(a, ln, mn),
+                    cell_rz(a, ln, mn)};
+}
+
+// The boundary geometry table's entries of cell (l, m) (lf_device.h).
+struct RingGeo {
+    double v[RG_N];
+};
+__device__ __forceinline__ void load_ring_geo(const StepArgs& a, int l, int m, RingGeo& g) {
+    const int64_t n = ring_lanes_compact(a.L, a.Mn);
+    const int c = ring_compact_index(l, m, a.L, a.Mn);
+#pragma unroll
+    for (int e = 0; e < RG_N; ++e) g.v[e] = a.ringgeo[e * n + c];
+}
+
 // Apply the segments [0, seg_end) of face f that cover cell (l, m) to the
 // physical state in the work area's `cur` slots, with the neighbour state in
 // its `nbr` slots: PerFieldBC::Apply's per-cell physical part
 // (src/bcs/per_field_bc.cpp:297-360, 391-446).  Returns the index of the last
 // segment applied, or -1.  The condition loop stays rolled: one copy of the
 // interpreter per call site.
-__device__ __forceinline__ int face_chain(const StepArgs& a, const WorkArea& w, int f, int l, int m, int seg_end) {
+__device__ __forceinline__ int face_chain(const StepArgs& a, const WorkArea& w, int f, int l, int m, int seg_end,
+                                          const ChainGeo& cg) {
     const BcImage* bc = img(a);
     const bool is_l = (f == MPD_L_LO || f == MPD_L_HI);
     const int i = is_l ? m : l;
-    // The geometry symbols, read before the segment loop so that their loads
-    // overlap the reads of the program.
-    int ln, mn;
-    face_nb(a, f, l, m, ln, mn);
-    const double g_dr = a.dr[l], g_r = cell_r(a, l, m), g_rz = cell_rz(a, l, m);
-    const double g_rnb = cell_r(a, ln, mn), g_rznb = cell_rz(a, ln, mn);
+    const double g_dr = cg.dr, g_r = cg.r, g_rz = cg.rz;
+    const double g_rnb = cg.rnb, g_rznb = cg.rznb;
     int last = -1;
     for (int s = 0; s < seg_end; ++s) {
         const MpdSeg& g = bc->seg(f, s);
@@ -728,7 +758,7 @@ __device__ __forceinline__ int face_chain(const StepArgs& a, const WorkArea& w, 
             w.sym(MPD_BS_DR) = g_dr;
             w.sym(MPD_BS_R) = g_r;
             w.sym(MPD_BS_RZ) = g_rz;
-            w.sym(MPD_BS_Z) = (double)(l - 1) * a.dz;
+            w.sym(MPD_BS_Z) = cg.z;
             w.sym(MPD_BS_RNB) = g_rnb;
             w.sym(MPD_BS_RZNB) = g_rznb;
 #pragma unroll
@@ -803,7 +833,7 @@ __device__ __forceinline__ void mlo_finish(const StepArgs& a, const WorkArea& w,
         src_phys(p.n, p.vn, cell_r(a, p.nl, p.nm), a.gm1, nb);
         w.put(WA_CUR, ph);
         w.put(WA_NBR, nb);
-        face_chain(a, w, p.cf, p.l, 1, p.ce);
+        face_chain(a, w, p.cf, p.l, 1, p.ce, chain_geo_rows(a, p.cf, p.l, 1));
         w.get(WA_CUR, ph);
     }
 }
@@ -858,9 +888,10 @@ __device__ __forceinline__ void axis_overrides(const StepArgs& a, const MpdSeg& 
 }
 
 // Final conserved values of boundary cell (l, m) written by face f (the last
-// face in L_LO, M_HI, M_LO, L_HI order that covers it).
+// face in L_LO, M_HI, M_LO, L_HI order that covers it); G: the cell's
+// boundary geometry table entries.
 __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, int f, int l, int m, double dt,
-                                         double* u) {
+                                         double* u, const RingGeo& G) {
     // Neighbour of the face (src/face_bc.cpp order: L_LO, M_HI, M_LO, L_HI;
     // only corners see an earlier face at their neighbour).
     int ln, mn;
@@ -881,22 +912,35 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
     src_load(so, vo);
     src_load(sn, vn);
     if (pf >= 0) src_load(sq, vq);
+    // The face neighbour is the row neighbour for an L face, the column
+    // neighbour for an M face; an earlier face's chain at a corner's
+    // neighbour reads the diagonal cell (table entries, lf_device.h).
+    const bool lf = (f == MPD_L_LO || f == MPD_L_HI);
+    const double r = G.v[RG_R];
+    const double r_nb = lf ? G.v[RG_R_ROW] : G.v[RG_R_COL], rz_nb = lf ? G.v[RG_RZ_ROW] : G.v[RG_RZ_COL];
     double st[10], nb[10];
-    src_phys(so, vo, cell_r(a, l, m), a.gm1, st);
-    src_phys(sn, vn, cell_r(a, ln, mn), a.gm1, nb);
+    src_phys(so, vo, r, a.gm1, st);
+    src_phys(sn, vn, r_nb, a.gm1, nb);
     if (pf >= 0) {
         double nb2[10];
-        src_phys(sq, vq, cell_r(a, ql, qm), a.gm1, nb2);
+        src_phys(sq, vq, G.v[RG_R_DIAG], a.gm1, nb2);
         w.put(WA_CUR, nb);
         w.put(WA_NBR, nb2);
-        face_chain(a, w, pf, pl, pm, img(a)->nseg[pf]);
+        // M_HI's corner (1, Mn): L_LO's chain at (1, Mn - 1), the column
+        // neighbour; L_HI's corners: M_LO's / M_HI's at (L - 1, m), the row
+        // neighbour.  Either way the chain's own neighbour is the diagonal.
+        const ChainGeo pg = (f == MPD_M_HI)
+            ? ChainGeo{G.v[RG_DR], G.v[RG_R_COL], G.v[RG_RZ_COL], G.v[RG_Z], G.v[RG_R_DIAG], G.v[RG_RZ_DIAG]}
+            : ChainGeo{G.v[RG_DR_ROW], G.v[RG_R_ROW], G.v[RG_RZ_ROW], G.v[RG_Z_ROW], G.v[RG_R_DIAG],
+                       G.v[RG_RZ_DIAG]};
+        face_chain(a, w, pf, pl, pm, img(a)->nseg[pf], pg);
         w.get(WA_CUR, nb);
     }
     w.put(WA_CUR, st);
     w.put(WA_NBR, nb);
-    const int last = face_chain(a, w, f, l, m, img(a)->nseg[f]);
+    const int last = face_chain(a, w, f, l, m, img(a)->nseg[f],
+                                ChainGeo{G.v[RG_DR], r, G.v[RG_RZ], G.v[RG_Z], r_nb, rz_nb});
     w.get(WA_CUR, st);
-    const double r = cell_r(a, l, m);
     rebuild_u(st, r, u);
     if (f == MPD_M_LO && last >= 0 && img(a)->seg(f, last).has_axis) {
         const MpdSeg& g = img(a)->seg(f, last);
@@ -939,6 +983,9 @@ __device__ void finalize_step(const StepArgs& a) {
     st->done = (!(st->t < a.T) || cap) ? 1 : 0;
     st->speed_bits = 0;
     st->ticket = 0;
+    // A new state outside the split division's range: the loop idles after
+    // this (complete) step until the host switches to IEEE division.
+    if (__hip_atomic_load(&st->range_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) st->range_stop = 1;
 }
 
 // Copies the program image (words 8-byte words) into `prog` (LDS).  Every
@@ -969,17 +1016,33 @@ __device__ __forceinline__ void stage_image(const uint64_t* src, uint64_t* prog,
 // area are the block's LDS.  finish: the last block to complete finishes the
 // loop step (ticket and device-scope fences); the batched path finishes its
 // instances' steps in a kernel of its own instead (lf_finish_batch).
-__device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt, int loop, int speed, int bid,
-                                              int nblk, const uint64_t* prog, double* area, bool finish,
-                                              bool grouped) {
+// A boundary lane's cell and its geometry table entries, read before the
+// program image is staged (the loads share the staging's round trip).
+struct RingLane {
+    bool valid;
+    int l, m;
+    RingGeo G;
+};
+
+__device__ __forceinline__ RingLane ring_lane(const StepArgs& a, int idx, bool grouped) {
+    RingLane r;
+    r.l = r.m = 0;
+    r.valid = grouped ? ring_cell(idx, a.L, a.Mn, r.l, r.m) : ring_cell_compact(idx, a.L, a.Mn, r.l, r.m);
+    if (r.valid) load_ring_geo(a, r.l, r.m, r.G);
+    return r;
+}
+
+__device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt, int loop, int speed, int nblk,
+                                              const uint64_t* prog, double* area, bool finish, const RingLane& rl) {
     StepArgs a = a_in;
     a.bc = prog;  // the staged image, read through img(a)
     const WorkArea w{area, (int)threadIdx.x};
     const int L = a.L, Mn = a.Mn;
-    const int idx = bid * BC_THREADS + (int)threadIdx.x;
     uint64_t key = 0;
-    int l = 0, m = 0;
-    if (grouped ? ring_cell(idx, L, Mn, l, m) : ring_cell_compact(idx, L, Mn, l, m)) {
+    bool rbad = false;
+    const int l = rl.l, m = rl.m;
+    if (rl.valid) {
+        const RingGeo& G = rl.G;
         // Last face covering the cell (M_LO skips l = 1 and l = L by its corner
         // policy, so it never owns a row-1 or row-L cell).
         int f = -1;
@@ -997,7 +1060,7 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt, i
         }
         double u[8];
         if (f >= 0) {
-            bc_final(a, w, f, l, m, dt, u);
+            bc_final(a, w, f, l, m, dt, u, G);
         } else if (m >= 2 && m <= Mn - 1) {
             // Uncovered row cell keeps its central value.
             const double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
@@ -1011,12 +1074,16 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt, i
             const int64_t i = at(a, l, m);
             for (int k = 0; k < 8; ++k) a.u_out[(int64_t)k * a.fstride + i] = u[k];
         }
-        if (speed) key = speed_key(wave_speed_u(u, cell_r(a, l, m), a.gamma, a.gm1));
+        if (speed) key = speed_key(wave_speed_u(u, G.v[RG_R], a.gamma, a.gm1));
+        // An uncovered column cell holds u = 0 (never read by a division
+        // with a nonzero numerator: its u_1 is 0, the state has diverged).
+        rbad = (f >= 0 || (m >= 2 && m <= Mn - 1)) && !split_div_safe(u);
     }
     if (speed) {
         key = wave_max_u64(key);
         if ((threadIdx.x & 63) == 0 && key != 0) atomicMax((unsigned long long*)&a.st->speed_bits, (unsigned long long)key);
     }
+    flag_range(a.st, rbad);
     if (!loop || !finish) return;
     // Last block finalises the loop state.
     __shared__ bool last;
@@ -1052,10 +1119,11 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring(StepArgs a_in, double d
     extern __shared__ uint64_t bc_lds[];
     const bool idle = loop && loop_idle(a_in);
     const double dt = step_dt(a_in, dt_explicit, loop);
+    const RingLane rl = ring_lane(a_in, (int)blockIdx.x * BC_THREADS + (int)threadIdx.x, true);
     stage_image(a_in.bc, bc_lds, a_in.bc_words);
     if (idle) return;
-    bc_ring_block(a_in, dt, loop, speed, (int)blockIdx.x, (int)gridDim.x, bc_lds,
-                  reinterpret_cast<double*>(bc_lds + a_in.bc_words), true, true);
+    bc_ring_block(a_in, dt, loop, speed, (int)gridDim.x, bc_lds, reinterpret_cast<double*>(bc_lds + a_in.bc_words),
+                  true, rl);
 }
 
 // The boundary kernel over many instances: work entry b = (instance, block
@@ -1072,10 +1140,10 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring_batch(const StepArgs* _
     extern __shared__ uint64_t bc_lds[];
     const bool idle = loop_idle(a);
     const double dt = step_dt(a, 0.0, 1);
+    const RingLane rl = ring_lane(a, w.first * BC_THREADS + (int)threadIdx.x, false);
     stage_image(a.bc, bc_lds, a.bc_words);
     if (idle) return;
-    bc_ring_block(a, dt, 1, speed, w.first, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words), false,
-                  false);
+    bc_ring_block(a, dt, 1, speed, w.count, bc_lds, reinterpret_cast<double*>(bc_lds + img_words), false, rl);
 }
 
 // The step's end for every instance of a batch that took it (after the

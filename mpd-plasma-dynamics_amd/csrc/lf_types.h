@@ -83,4 +83,6 @@ struct MpdLoopState {
     uint32_t ticket2;
     int32_t checked_step;  // step at which SolutionChange last ran in the loop
     int32_t fallback;      // 1: the candidate max is not provably the max; rescan
+    uint32_t range_bits;   // nonzero: a value of the new state is outside split_div_safe's range
+    int32_t range_stop;    // 1: the loop idles until the host switches to IEEE division
 };

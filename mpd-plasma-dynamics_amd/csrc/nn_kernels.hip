@@ -273,6 +273,140 @@ __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
             }
 }
 
+// The same implicit GEMM for channel counts that are multiples of 8, on
+// gfx950's v_mfma_f32_16x16x32_bf16 (K = 32 per instruction): a 256-thread
+// workgroup computes a TM x TN tile of C (up to 128 x 128; the narrow
+// shapes serve the output head and the small channel counts), the four
+// waves in a WM x WN arrangement, each a (TM / WM) x (TN / WN) sub-tile of
+// 16 x 16 accumulators.  Operand tiles of TM x 32 and TN x 32 go through
+// two LDS buffers: the global loads of k-tile j + 1 are issued before the
+// matrix instructions of tile j and stored after them, into the other
+// buffer, so one barrier per k-tile separates the two.  Lane l of an MFMA
+// holds row l % 16 and the 8 k values 8 (l / 16) .. + 7 of each operand;
+// its results are C[4 (l / 16) + i][l % 16], i < 4.
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+constexpr int BK2 = 32, LDK2 = BK2 + 8;  // LDS row pitch (bf16): 80 bytes
+
+// Run r of a T x 32 operand tile: along k, row r / 4, columns 8 (r % 4) ..
+// + 7; along rows, rows 8 (r % (T / 8)) .. + 7 and column r / (T / 8).
+template <int T, bool kRowsFast, class Load>
+__device__ __forceinline__ uint4 stage_load2(int r, int r0, int k0, int k_end, Load ld) {
+    if (kRowsFast) {
+        const int k = k0 + r / (T / 8);
+        return (k < k_end) ? ld(r0 + 8 * (r % (T / 8)), k) : zero_run();
+    }
+    const int k = k0 + 8 * (r & 3);
+    return (k < k_end) ? ld(r0 + (r >> 2), k) : zero_run();
+}
+
+template <int T, bool kRowsFast>
+__device__ __forceinline__ void stage_store2(int r, uint16_t* S, uint4 v) {
+    if (kRowsFast) {
+        Run8 u;
+        u.v = v;
+        const int c = r / (T / 8), row = 8 * (r % (T / 8));
+#pragma unroll
+        for (int j = 0; j < 8; ++j) S[(row + j) * LDK2 + c] = u.h[j];
+    } else {
+        *reinterpret_cast<uint4*>(&S[(r >> 2) * LDK2 + 8 * (r & 3)]) = v;
+    }
+}
+
+template <int MODE, int TM, int TN, int WM, int WN>
+__global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
+    static_assert(WM * WN == 4 && TM % (16 * WM) == 0 && TN % (16 * WN) == 0, "tile shape");
+    constexpr int FM = TM / WM / 16, FN = TN / WN / 16;  // 16 x 16 blocks per wave
+    constexpr int RA = (TM * 4 + 255) / 256, RB = (TN * 4 + 255) / 256;  // staging runs per thread
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * LDK2];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][TN * LDK2];
+    const int t = (int)threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int m0 = (int)blockIdx.x * TM, n0 = (int)blockIdx.y * TN;
+    int k_begin = 0, k_end = a.K;
+    if (MODE == MODE_WGRAD) {
+        k_begin = (int)blockIdx.z * a.kchunk;
+        k_end = min(a.K, k_begin + a.kchunk);
+    }
+    f32x4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto lda = [&](int m, int k) { return vload_a<MODE>(a, m, k); };
+    auto ldb = [&](int n, int k) { return vload_b<MODE>(a, n, k); };
+    uint4 ra[RA], rb[RB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < RA; ++q) {
+            const int r = t + 256 * q;
+            ra[q] = (r < TM * 4) ? stage_load2<TM, a_rows_fast<MODE>()>(r, m0, k0, k_end, lda) : zero_run();
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int r = t + 256 * q;
+            rb[q] = (r < TN * 4) ? stage_load2<TN, b_rows_fast<MODE>()>(r, n0, k0, k_end, ldb) : zero_run();
+        }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < RA; ++q) {
+            const int r = t + 256 * q;
+            if (r < TM * 4) stage_store2<TM, a_rows_fast<MODE>()>(r, As[buf], ra[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int r = t + 256 * q;
+            if (r < TN * 4) stage_store2<TN, b_rows_fast<MODE>()>(r, Bs[buf], rb[q]);
+        }
+    };
+    const int row = lane & 15, kq = (lane >> 4) * 8;
+    int buf = 0;
+    if (k_begin < k_end) {
+        fetch(k_begin);
+        put(0);
+    }
+    __syncthreads();
+    for (int k0 = k_begin; k0 < k_end; k0 += BK2) {
+        const bool more = k0 + BK2 < k_end;
+        if (more) fetch(k0 + BK2);
+        bf16x8v fa[FM], fb[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+            fa[i] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const uint4*>(
+                                                    &As[buf][((wm * FM + i) * 16 + row) * LDK2 + kq]));
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+            fb[j] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const uint4*>(
+                                                    &Bs[buf][((wn * FN + j) * 16 + row) * LDK2 + kq]));
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        if (more) put(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    float* out = a.out;
+    if (MODE == MODE_WGRAD) out += (int64_t)blockIdx.z * a.M * a.N;
+    const int kr = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int m = m0 + (wm * FM + i) * 16 + kr + jj;
+                const int n = n0 + (wn * FN + j) * 16 + row;
+                if (m < a.M && n < a.N) {
+                    float v = acc[i][j][jj];
+                    if (MODE == MODE_FWD && a.bias) v += a.bias[n];
+                    out[(int64_t)m * a.N + n] = v;
+                }
+            }
+}
+
 // out[i] = sum over rows s < S of part[s][i], i < n: a block of 16 waves
 // per 64 outputs, wave w summing rows w, w + 16, ..., then the 16 partial
 // sums added in a fixed tree order (deterministic, whatever the hardware
@@ -517,14 +651,61 @@ int bias_slices(int P, int C) {
     return std::max(1, std::min(1024, P / 128));
 }
 
+// Tile of the matrix-core GEMM for C = M x N (channel counts multiples of
+// 8): 128 x N-wide tiles for the pixel-major passes, Cout-high tiles for the
+// weight gradient.
+struct Tile {
+    int tm, tn;
+};
+Tile pick_tile(int M, int N) {
+    Tile t;
+    t.tn = N <= 16 ? 16 : N <= 32 ? 32 : N <= 64 ? 64 : 128;
+    t.tm = M <= 16 ? 16 : M <= 32 ? 32 : M <= 64 ? 64 : 128;
+    if (t.tm == 16 && t.tn < 64) t.tn = 64;  // four waves need 64 lanes of N then
+    return t;
+}
+bool vec_ok(int Cin, int Cout) { return Cin % 8 == 0 && Cout % 8 == 0; }
+
 // Pixel splits of the weight gradient: enough workgroups to fill the GPU
 // (about 2048 over the output tiles), each split at least 256 pixels.
-int wgrad_splits(int P, int M, int N) {
-    const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+int wgrad_splits(int P, int M, int N, bool vec) {
+    int tm = BM, tn = BN;
+    if (vec) {
+        const Tile t = pick_tile(M, N);
+        tm = t.tm;
+        tn = t.tn;
+    }
+    const int tiles = ((M + tm - 1) / tm) * ((N + tn - 1) / tn);
     const int want = (2048 + tiles - 1) / tiles;
     return std::max(1, std::min(want, P / 256));
 }
 int wgrad_chunk(int P, int S) { return ((P + S - 1) / S + BK - 1) / BK * BK; }
+
+// Launch of conv_gemm2 for C = a.M x a.N with the tile pick_tile chooses.
+template <int MODE>
+void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
+    const Tile t = pick_tile(a.M, a.N);
+    const dim3 g((unsigned)((a.M + t.tm - 1) / t.tm), (unsigned)((a.N + t.tn - 1) / t.tn), (unsigned)splits);
+#define MPD_GEMM2(TM, TN, WM, WN) \
+    if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm2<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return; }
+    MPD_GEMM2(128, 128, 2, 2)
+    MPD_GEMM2(128, 64, 2, 2)
+    MPD_GEMM2(128, 32, 4, 1)
+    MPD_GEMM2(128, 16, 4, 1)
+    MPD_GEMM2(64, 128, 2, 2)
+    MPD_GEMM2(64, 64, 2, 2)
+    MPD_GEMM2(64, 32, 4, 1)
+    MPD_GEMM2(64, 16, 4, 1)
+    MPD_GEMM2(32, 128, 2, 2)
+    MPD_GEMM2(32, 64, 2, 2)
+    MPD_GEMM2(32, 32, 2, 2)
+    MPD_GEMM2(16, 128, 1, 4)
+    MPD_GEMM2(16, 64, 1, 4)
+#undef MPD_GEMM2
+    // Any other shape (every access is bounds-checked): the widest tile.
+    const dim3 g2((unsigned)((a.M + 127) / 128), (unsigned)((a.N + 127) / 128), (unsigned)splits);
+    hipLaunchKernelGGL((conv_gemm2<MODE, 128, 128, 2, 2>), g2, dim3(256), 0, s, a);
+}
 
 }  // namespace
 
@@ -570,7 +751,7 @@ int mpd_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, 
     a.N = Cout;
     a.K = 9 * Cin;
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
-    if (Cin % 8 == 0) hipLaunchKernelGGL((conv_gemm<MODE_FWD, true>), g, dim3(256), 0, (hipStream_t)stream, a);
+    if (vec_ok(Cin, Cout)) launch_gemm2<MODE_FWD>(a, 1, (hipStream_t)stream);
     else hipLaunchKernelGGL((conv_gemm<MODE_FWD, false>), g, dim3(256), 0, (hipStream_t)stream, a);
     return finish(hipSuccess);
 }
@@ -587,8 +768,7 @@ int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, in
     a.N = Cin;
     a.K = 9 * Cout;
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
-    if (Cin % 8 == 0 && Cout % 8 == 0)
-        hipLaunchKernelGGL((conv_gemm<MODE_DGRAD, true>), g, dim3(256), 0, (hipStream_t)stream, a);
+    if (vec_ok(Cin, Cout)) launch_gemm2<MODE_DGRAD>(a, 1, (hipStream_t)stream);
     else hipLaunchKernelGGL((conv_gemm<MODE_DGRAD, false>), g, dim3(256), 0, (hipStream_t)stream, a);
     return finish(hipSuccess);
 }
@@ -596,7 +776,7 @@ int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, in
 long long mpd_nn_wgrad_work_floats(int B, int H, int W, int Cin, int Cout) {
     if (!conv_dims_ok(B, H, W, Cin, Cout)) return -1;
     const int P = B * H * W;
-    return (long long)wgrad_splits(P, Cout, 9 * Cin) * Cout * 9 * Cin;
+    return (long long)wgrad_splits(P, Cout, 9 * Cin, vec_ok(Cin, Cout)) * Cout * 9 * Cin;
 }
 
 int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, float* work, int B, int H, int W,
@@ -610,11 +790,12 @@ int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, 
     a.M = Cout;
     a.N = 9 * Cin;
     a.K = B * H * W;
-    const int S = wgrad_splits(a.K, a.M, a.N);
+    const bool vec = vec_ok(Cin, Cout);
+    const int S = wgrad_splits(a.K, a.M, a.N, vec);
     a.kchunk = wgrad_chunk(a.K, S);
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN), (unsigned)S);
     hipStream_t s = (hipStream_t)stream;
-    if (Cin % 8 == 0 && Cout % 8 == 0) hipLaunchKernelGGL((conv_gemm<MODE_WGRAD, true>), g, dim3(256), 0, s, a);
+    if (vec) launch_gemm2<MODE_WGRAD>(a, S, s);
     else hipLaunchKernelGGL((conv_gemm<MODE_WGRAD, false>), g, dim3(256), 0, s, a);
     const int64_t n = (int64_t)a.M * a.N;
     hipLaunchKernelGGL(column_sum, dim3((unsigned)((n + 63) / 64)), dim3(1024), 0, s, (const float*)work, dw, n, S);

@@ -218,6 +218,7 @@ struct mpd_solver {
     double* rowsum = nullptr;
     double* scratch = nullptr;
     double* geo = nullptr;
+    double* ringgeo = nullptr;  // boundary geometry table (RingGeo layout, lf_device.h)
     uint64_t* bc = nullptr;  // the BC program's image (BcImage layout)
     MpdLoopState* st = nullptr;
     unsigned long long* umax = nullptr;
@@ -248,7 +249,7 @@ struct mpd_solver {
     ~mpd_solver() {
         for (auto& g : graph)
             if (g) (void)hipGraphExecDestroy(g);
-        double* bufs[] = {state[0], state[1], rowbuf, phys_ic, prev, partial, rowsum, scratch, geo};
+        double* bufs[] = {state[0], state[1], rowbuf, phys_ic, prev, partial, rowsum, scratch, geo, ringgeo};
         for (double* b : bufs)
             if (b) (void)hipFree(b);
         if (bc) (void)hipFree(bc);
@@ -293,7 +294,30 @@ struct mpd_solver {
         a.bc_area = bc_area;
         a.st = st;
         a.bc = bc;
+        a.ringgeo = ringgeo;
         return a;
+    }
+
+    // The boundary geometry table: for every boundary cell in compact order,
+    // r and r_z of the cell, its row and column neighbours and the diagonal,
+    // dr and z of its row and of the row neighbour's (the host's HostGrid
+    // values, bit-identical to the device's cell_r / cell_rz).
+    void UploadRingGeo() {
+        const int L = grid.L, Mn = grid.Mn;
+        const int n = mpd::ring_lanes_compact(L, Mn);
+        std::vector<double> t((size_t)mpd::RG_N * n);
+        for (int c = 0; c < n; ++c) {
+            int l = 0, m = 0;
+            mpd::ring_cell_compact(c, L, Mn, l, m);
+            const int lr = (l == 1) ? 2 : (l == L ? L - 1 : l);
+            const int mc = (m == 1) ? 2 : (m == Mn ? Mn - 1 : m);
+            const double v[mpd::RG_N] = {grid.r(l, m),   grid.r_z(l, m),   grid.dr[l],      (double)(l - 1) * cfg.dz,
+                                         grid.r(lr, m),  grid.r_z(lr, m),  grid.dr[lr],     (double)(lr - 1) * cfg.dz,
+                                         grid.r(l, mc),  grid.r_z(l, mc),  grid.r(lr, mc),  grid.r_z(lr, mc)};
+            for (int e = 0; e < mpd::RG_N; ++e) t[(size_t)e * n + c] = v[e];
+        }
+        ringgeo = Alloc(t.size());
+        HIPCHK(hipMemcpy(ringgeo, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
     }
 
     double* Alloc(size_t ndouble) {
@@ -425,6 +449,7 @@ struct mpd_solver {
             HIPCHK(hipMemcpy(bc, image.data(), image.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
         }
         bc_area = mpd::bc_area_slots(hbc);
+        UploadRingGeo();
         if (ext_st) {
             st = ext_st;
             own_st = false;
@@ -465,8 +490,52 @@ struct mpd_solver {
         speed_src = 0;
     }
 
+    // Host form of split_div_safe (lf_device.h) over the interior of host
+    // arrays u0 (rows x cols each), and the radius range it assumes.
+    static bool HostExpOk(double v, int lo, int hi) {
+        int e = 0;
+        if (v == 0.0) return true;
+        if (!std::isfinite(v)) return true;  // a non-finite state ends the loop anyway
+        (void)std::frexp(v, &e);
+        return e >= lo && e <= hi;
+    }
+    bool StateInSplitRange(const double* const* u0) const {
+        for (int l = 1; l <= grid.L; ++l)
+            for (int m = 1; m <= grid.Mn; ++m) {
+                const size_t i = (size_t)l * cols + m;
+                if (!HostExpOk(u0[0][i], -29, 31) || u0[0][i] == 0.0) return false;
+                for (int k = 1; k < 8; ++k)
+                    if (!HostExpOk(u0[k][i], -99, 61)) return false;
+            }
+        return true;
+    }
+    bool RadiiInSplitRange() const {
+        for (int l = 1; l <= grid.L; ++l)
+            for (int m = 1; m <= grid.Mn; ++m) {
+                const double r = grid.r(l, m);
+                if (!(r >= 0.00390625 && r <= 256.0)) return false;
+            }
+        return true;
+    }
+    // The split division is used only while every state stays in the range
+    // where it equals IEEE division; from the step after one leaves it, the
+    // solver runs with IEEE division (fast_div_step records when).
+    int fast_div_step = -1;
+    void SwitchToIeeeDivision() {
+        if (!scfg.fast_div) return;
+        scfg.fast_div = false;
+        fast_div_step = step;
+        for (auto& g : graph)
+            if (g) {
+                HIPCHK(hipGraphExecDestroy(g));
+                g = nullptr;
+            }
+        graph_K = 0;
+    }
+
     void UploadState(const double* const* u0, const double* const* phys) {
         HIPCHK(hipStreamSynchronize(stream));
+        if (scfg.fast_div && !(RadiiInSplitRange() && StateInSplitRange(u0))) SwitchToIeeeDivision();
         UploadFields(state[cur], u0, 8);
         ZeroPitched(state[1 - cur], (size_t)8 * rows);
         if (phys) {
@@ -480,6 +549,17 @@ struct mpd_solver {
     }
 
     void Advance(double dt_step) {
+        if (scfg.fast_div) {
+            // The previous step's kernels flag a new state outside the split
+            // division's range (range_bits): this step then divides exactly.
+            uint32_t bits = 0;
+            HIPCHK(hipMemcpyAsync(&bits, &st->range_bits, sizeof(bits), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            if (bits) {
+                SwitchToIeeeDivision();
+                HIPCHK(hipMemsetAsync(&st->range_bits, 0, sizeof(uint32_t), stream));
+            }
+        }
         HIPCHK(hipMemsetAsync(&st->speed_bits, 0, sizeof(uint64_t), stream));
         const StepArgs a = Args(cur);
         HIPCHK(mpd::launch_stencil(a, scfg, dt_step, 0, 1, stream));
@@ -681,6 +761,19 @@ struct mpd_solver {
         scfg.tile_rows = best;
     }
 
+    // Steps this call may still take (max_steps <= 0: no cap of its own).
+    static int Remaining(int max_steps, int taken) { return max_steps > 0 ? max_steps - taken : 0; }
+
+    // After a step whose new state left the split division's range (the loop
+    // idles behind range_stop): IEEE division from here, the loop re-seeded.
+    // Returns whether the loop has ended.
+    bool AfterRangeStop(const MpdLoopState& h, int max_steps, int step0) {
+        SwitchToIeeeDivision();
+        if (h.done) return true;
+        if (max_steps > 0 && step - step0 >= max_steps) return true;
+        return SeedLoop(Remaining(max_steps, step - step0));
+    }
+
     void Run(int max_steps, mpd_run_result* out) {
         const int step0 = step;
         const bool conv = cfg.convergence_threshold > 0.0;
@@ -691,10 +784,10 @@ struct mpd_solver {
             EnqueueLoopStep(cur, conv, true);
             const MpdLoopState h = ReadLoop();
             Absorb(h);
-            done = h.done != 0;
+            done = h.range_stop ? AfterRangeStop(h, max_steps, step0) : (h.done != 0);
         }
-        const int K = ChooseK();
         while (!done) {
+            const int K = ChooseK();
             if (!graph[0] || graph_K != K) CaptureGraphs(K, conv);
             HIPCHK(hipGraphLaunch(graph[cur], stream));
             MpdLoopState h = ReadLoop();
@@ -704,7 +797,7 @@ struct mpd_solver {
                 h = ReadLoop();
                 Absorb(h);
             }
-            done = h.done != 0;
+            done = h.range_stop ? AfterRangeStop(h, max_steps, step0) : (h.done != 0);
         }
         if (out) {
             out->t = t;
@@ -715,6 +808,7 @@ struct mpd_solver {
             out->converged = converged ? 1 : 0;
             // The loop's own end (src/main.cpp:219), not this call's step cap.
             out->done = (!(t < cfg.T) || converged || (cfg.max_steps > 0 && step >= cfg.max_steps)) ? 1 : 0;
+            out->fast_div = scfg.fast_div ? 1 : 0;
         }
     }
 

@@ -29,7 +29,7 @@ class MpdError(RuntimeError):
 
 class RunResult(C.Structure):
     _fields_ = [("t", C.c_double), ("dt", C.c_double), ("change", C.c_double), ("steps", C.c_int),
-                ("total_steps", C.c_int), ("converged", C.c_int), ("done", C.c_int)]
+                ("total_steps", C.c_int), ("converged", C.c_int), ("done", C.c_int), ("fast_div", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -477,7 +477,7 @@ def load_network(path, L, M, seed=2024):
 
 
 def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, clip=None, plain_sample=16,
-               seed=2024, out="", warm_levels=None, ref_threshold=None, load=""):
+               seed=2024, out="", warm_levels=None, ref_threshold=None, load="", gen_batch=512):
     """The whole experiment: train_n steady states generated, the network
     trained on them (or a saved one loaded), test_n held-out cases evaluated
     (evaluate); prints the result as JSON and, with `out`, writes it and the
@@ -488,7 +488,7 @@ def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, cli
         progress(f"network loaded from {load}")
     else:
         t0 = time.perf_counter()
-        params, fields, steps, conv = generate(train_n, L, M, seed)
+        params, fields, steps, conv = generate(train_n, L, M, seed, batch=gen_batch)
         gen_s = time.perf_counter() - t0
         keep = conv
         progress(f"generated {train_n} steady states in {gen_s:.1f}s ({keep.sum()} converged, steps p50 "
@@ -516,6 +516,14 @@ def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, cli
             np.savez(os.path.join(out, "warmstart_net.npz"), **{f"w{k}": w for k, w in enumerate(sd["w"])},
                      **{f"b{k}": b for k, b in enumerate(sd["b"])}, mean=norm.mean, std=norm.std,
                      train_steps=train_steps)
+    if test_n <= 0:
+        # Training only (the network is in `out` for a later evaluation).
+        result_extra["n_params"] = net.n_params()
+        print(json.dumps(result_extra, indent=1), flush=True)
+        if out:
+            with open(os.path.join(out, "warmstart_train.json"), "w") as f:
+                json.dump(result_extra, f, indent=1)
+        return result_extra
     test_params = sweep.sample_params(test_n, seed + 1)
     result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=plain_sample,
                             budget=step_budget(train_steps), warm_levels=warm_levels, ref_threshold=ref_threshold)

@@ -324,3 +324,40 @@ def test_run_done_flag(mpd):
         assert r["steps"] == 10 and r["done"] == 0
         r = sol.run()
         assert r["steps"] == 20 and r["total_steps"] == 30 and r["done"] == 1
+
+
+def _with_ic_vphi(text, value):
+    old = "  v_r: 0.0\n  v_phi: 0.0\n  H_z: 0.0"
+    assert text.count(old) == 1
+    return text.replace(old, f"  v_r: 0.0\n  v_phi: {value}\n  H_z: 0.0")
+
+
+@pytest.mark.parametrize("vphi,label", [("1.0e-40", "IC outside the range"),
+                                        ("2.0e-29", "state leaves the range during the run")])
+def test_split_division_range_switch(mpd, vphi, label):
+    """The split division equals IEEE division only while the state stays in
+    its range (|u_1| in [2^-30, 2^31), other conserved values zero or in
+    [2^-100, 2^61)).  A tiny azimuthal velocity puts u_4 below 2^-100: at
+    upload (1e-40), or after a few steps as the zero inflow value eats into
+    a 2e-29 field.  The solver then runs with IEEE division -- the result
+    reports it -- and stays bit-identical to the oracle; the same case
+    batched with an ordinary one switches the batch."""
+    text = _with_ic_vphi(cases.channel_config(40, 16, geometry="condiv", noise=1e-3, max_steps=60), vphi)
+    o, ores = run_oracle(text, 0)
+    with mpd.Solver(text) as sol:
+        res = sol.run()
+        u, p = sol.download()
+    print(label, res, flush=True)
+    assert res["fast_div"] == 0
+    assert (res["steps"], res["t"], res["dt"]) == (ores["steps"], ores["t"], ores["dt"])
+    for k in range(8):
+        assert same_bits(u[k], o.u0(k)), f"u0_{k + 1}: {describe_mismatch(u[k], o.u0(k))}"
+    plain = cases.channel_config(30, 12, max_steps=60)
+    with mpd.Solver(plain) as sol:
+        assert sol.run()["fast_div"] == 1
+    with mpd.Batch([text, plain]) as b:
+        rb = b.run()
+        assert all(r["fast_div"] == 0 for r in rb)
+        ub, _ = b.instance(0).download(phys=False)
+    for k in range(8):
+        assert same_bits(ub[k], o.u0(k)), f"batched u0_{k + 1}"

@@ -51,7 +51,11 @@ def _empty(shape, bf16=False):
     return surrogate.DeviceArray(shape, "bf16" if bf16 else "f32")
 
 
-SHAPES = [(2, 8, 5, 3, 5), (3, 12, 10, 8, 32), (1, 16, 8, 96, 32), (2, 6, 5, 40, 70), (1, 5, 3, 256, 130)]
+# Channel counts that are multiples of 8 take the matrix-core GEMM with the
+# 16x16x32 instruction (tile shapes for N = 8, 16, 32, 64, 128, 256 and small
+# pixel counts); the others the element-staged kernel.
+SHAPES = [(2, 8, 5, 3, 5), (3, 12, 10, 8, 32), (1, 16, 8, 96, 32), (2, 6, 5, 40, 70), (1, 5, 3, 256, 130),
+          (2, 9, 7, 16, 8), (1, 20, 24, 64, 128), (4, 3, 2, 32, 256), (1, 4, 4, 128, 16), (2, 10, 6, 384, 128)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
