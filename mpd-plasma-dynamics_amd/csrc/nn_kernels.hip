@@ -312,11 +312,58 @@ __device__ __forceinline__ void stage_store2(int r, uint16_t* S, uint4 v) {
     }
 }
 
+// Index k = tap * C + c of a run along a (tap, channel) axis, advanced tile
+// by tile without division.
+struct TapChan {
+    int tap, c;
+    __device__ __forceinline__ void init(int k, int C) {
+        tap = k / C;
+        c = k - tap * C;
+    }
+    __device__ __forceinline__ void step(int dk, int C) {
+        c += dk;
+        while (c >= C) {
+            c -= C;
+            ++tap;
+        }
+    }
+};
+
+// A pixel index k = (b H + y) W + x, advanced tile by tile without division.
+struct PixPos {
+    int b, y, x;
+    __device__ __forceinline__ void init(int k, int H, int W) {
+        const int hw = H * W;
+        b = k / hw;
+        const int r = k - b * hw;
+        y = r / W;
+        x = r - y * W;
+    }
+    __device__ __forceinline__ void step(int dk, int H, int W) {
+        x += dk;
+        while (x >= W) {
+            x -= W;
+            if (++y >= H) {
+                y = 0;
+                ++b;
+            }
+        }
+    }
+};
+
+// The 16-byte run at channel c of pixel (b, y + dy, x + dx) of a [B][H][W][C]
+// tensor, zero outside the image.
+__device__ __forceinline__ uint4 pix_run(const uint16_t* t, const ConvArgs& a, int C, int b, int y, int x, int c) {
+    if (y < 0 || y >= a.H || x < 0 || x >= a.W) return zero_run();
+    return *reinterpret_cast<const uint4*>(t + ((int64_t)(b * a.H + y) * a.W + x) * C + c);
+}
+
 template <int MODE, int TM, int TN, int WM, int WN>
 __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
     static_assert(WM * WN == 4 && TM % (16 * WM) == 0 && TN % (16 * WN) == 0, "tile shape");
     constexpr int FM = TM / WM / 16, FN = TN / WN / 16;  // 16 x 16 blocks per wave
     constexpr int RA = (TM * 4 + 255) / 256, RB = (TN * 4 + 255) / 256;  // staging runs per thread
+    constexpr bool AR = a_rows_fast<MODE>(), BR = b_rows_fast<MODE>();
     __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * LDK2];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][TN * LDK2];
     const int t = (int)threadIdx.x;
@@ -333,31 +380,93 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    auto lda = [&](int m, int k) { return vload_a<MODE>(a, m, k); };
-    auto ldb = [&](int n, int k) { return vload_b<MODE>(a, n, k); };
+
+    // Staging runs: run r of a T x 32 tile lies along k at row r / 4,
+    // columns 8 (r % 4) .. + 7, or along rows (rows 8 (r % (T / 8)) .. + 7,
+    // column r / (T / 8)); its row and column are the same in every k-tile,
+    // so the gather positions are set up once and advanced by 32 per tile.
+    int a_row[RA], a_kc[RA], b_row[RB], b_kc[RB];
+    bool a_on[RA], b_on[RB];
+    PixPos a_pix[RA], b_pix[RB];
+    TapChan a_tc[RA], b_tc[RB];
+#pragma unroll
+    for (int q = 0; q < RA; ++q) {
+        const int r = t + 256 * q;
+        a_on[q] = r < TM * 4;
+        a_row[q] = AR ? m0 + 8 * (r % (TM / 8)) : m0 + (r >> 2);
+        a_kc[q] = AR ? r / (TM / 8) : 8 * (r & 3);
+        a_on[q] = a_on[q] && a_row[q] < a.M;
+        if (MODE == MODE_FWD || MODE == MODE_DGRAD) {
+            // A = input (forward) / dz (data gradient) at pixel row, shifted by the tap of k.
+            a_pix[q].init(a_on[q] ? a_row[q] : 0, a.H, a.W);
+            a_tc[q].init(k_begin + a_kc[q], MODE == MODE_FWD ? a.Cin : a.Cout);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        const int r = t + 256 * q;
+        b_on[q] = r < TN * 4;
+        b_row[q] = BR ? n0 + 8 * (r % (TN / 8)) : n0 + (r >> 2);
+        b_kc[q] = BR ? r / (TN / 8) : 8 * (r & 3);
+        b_on[q] = b_on[q] && b_row[q] < a.N;
+        if (MODE == MODE_DGRAD) b_tc[q].init(k_begin + b_kc[q], a.Cout);  // k = (tap, co)
+        if (MODE == MODE_WGRAD) {
+            b_tc[q].init(b_on[q] ? b_row[q] : 0, a.Cin);  // row = (tap, ci), fixed
+            b_pix[q].init(k_begin + b_kc[q], a.H, a.W);    // k = pixel
+        }
+    }
     uint4 ra[RA], rb[RB];
+    // Loads of the k-tile at k0 (called for k0 = k_begin, k_begin + 32, ...
+    // in order: each call advances the runs' positions to the next tile).
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < RA; ++q) {
-            const int r = t + 256 * q;
-            ra[q] = (r < TM * 4) ? stage_load2<TM, a_rows_fast<MODE>()>(r, m0, k0, k_end, lda) : zero_run();
+            const int k = k0 + a_kc[q];
+            uint4 v = zero_run();
+            if (a_on[q] && k < k_end) {
+                if (MODE == MODE_FWD) {
+                    const int tap = a_tc[q].tap, dy = tap / 3 - 1, dx = tap - 3 * (tap / 3) - 1;
+                    v = pix_run(a.x, a, a.Cin, a_pix[q].b, a_pix[q].y + dy, a_pix[q].x + dx, a_tc[q].c);
+                } else if (MODE == MODE_DGRAD) {
+                    const int tap = a_tc[q].tap, dy = 1 - tap / 3, dx = 1 - (tap - 3 * (tap / 3));
+                    v = pix_run(a.dz, a, a.Cout, a_pix[q].b, a_pix[q].y + dy, a_pix[q].x + dx, a_tc[q].c);
+                } else {
+                    v = *reinterpret_cast<const uint4*>(a.dz + (int64_t)k * a.Cout + a_row[q]);
+                }
+            }
+            ra[q] = v;
+            if (MODE != MODE_WGRAD) a_tc[q].step(BK2, MODE == MODE_FWD ? a.Cin : a.Cout);
         }
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
-            const int r = t + 256 * q;
-            rb[q] = (r < TN * 4) ? stage_load2<TN, b_rows_fast<MODE>()>(r, n0, k0, k_end, ldb) : zero_run();
+            const int k = k0 + b_kc[q];
+            uint4 v = zero_run();
+            if (b_on[q] && k < k_end) {
+                if (MODE == MODE_FWD) {
+                    v = *reinterpret_cast<const uint4*>(a.w + (int64_t)b_row[q] * a.K + k);
+                } else if (MODE == MODE_DGRAD) {
+                    v = *reinterpret_cast<const uint4*>(a.w + ((int64_t)b_tc[q].c * 9 + b_tc[q].tap) * a.Cin +
+                                                        b_row[q]);
+                } else {
+                    const int tap = b_tc[q].tap, dy = tap / 3 - 1, dx = tap - 3 * (tap / 3) - 1;
+                    v = pix_run(a.x, a, a.Cin, b_pix[q].b, b_pix[q].y + dy, b_pix[q].x + dx, b_tc[q].c);
+                }
+            }
+            rb[q] = v;
+            if (MODE == MODE_DGRAD) b_tc[q].step(BK2, a.Cout);
+            if (MODE == MODE_WGRAD) b_pix[q].step(BK2, a.H, a.W);
         }
     };
     auto put = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < RA; ++q) {
             const int r = t + 256 * q;
-            if (r < TM * 4) stage_store2<TM, a_rows_fast<MODE>()>(r, As[buf], ra[q]);
+            if (r < TM * 4) stage_store2<TM, AR>(r, As[buf], ra[q]);
         }
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
             const int r = t + 256 * q;
-            if (r < TN * 4) stage_store2<TN, b_rows_fast<MODE>()>(r, Bs[buf], rb[q]);
+            if (r < TN * 4) stage_store2<TN, BR>(r, Bs[buf], rb[q]);
         }
     };
     const int row = lane & 15, kq = (lane >> 4) * 8;

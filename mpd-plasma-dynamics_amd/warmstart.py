@@ -258,18 +258,22 @@ def set_threshold(b, n, level):
         b.instance(i).set_option("convergence_threshold", level)
 
 
-def run_to_end(b, n, chunk, steps, finish=None, t_start=None, on_chunk=None):
+def run_to_end(b, n, chunk, steps, finish=None, t_start=None, on_chunk=None, label="batch"):
     """Runs batch b until every instance's loop has ended, in slices of
     `chunk` steps; adds the steps taken to `steps` and records in `finish`
     (seconds since t_start) when each instance ended.  on_chunk(res) may
     restart instances (it returns the set it restarted).  Returns the last
     results and the seconds spent in the runs."""
     spent = 0.0
+    last = time.perf_counter()
     while True:
         t0 = time.perf_counter()
         res = b.run(chunk)
         spent += time.perf_counter() - t0
         steps += np.array([r["steps"] for r in res])
+        if time.perf_counter() - last >= 10.0:
+            last = time.perf_counter()
+            progress(f"{label}: {spent:.1f}s, {sum(r['done'] for r in res)}/{n} done, max steps {steps.max()}")
         if finish is not None:
             el = time.perf_counter() - t_start
             for i, r in enumerate(res):
@@ -292,14 +296,19 @@ def reference_states(cfgs, L, M, ref_threshold, max_steps, chunk=20000):
     b = mpd_amd.Batch(cfgs)
     b.autotune(5)
     steps = np.zeros(n, int)
-    run_to_end(b, n, chunk, steps)
+    run_to_end(b, n, chunk, steps, label="reference (cold to the sweep threshold)")
     set_threshold(b, n, ref_threshold)
     cap = np.zeros(n, int)
+    t0 = last = time.perf_counter()
     while True:
         res = b.run(chunk)
         for i, r in enumerate(res):
             steps[i] += r["steps"]
             cap[i] += r["steps"]
+        if time.perf_counter() - last >= 10.0:
+            last = time.perf_counter()
+            progress(f"reference to {ref_threshold:g}: {last - t0:.1f}s, {sum(r['done'] for r in res)}/{n} done, "
+                     f"max steps {steps.max()}")
         if all(r["done"] or cap[i] >= max_steps for i, r in enumerate(res)):
             break
     reached = np.array([bool(r["converged"]) for r in res])
@@ -394,14 +403,14 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
         return out
 
     level_fields, level_steps, level_wall = [], [], []
-    res, spent = run_to_end(b, n, chunk, steps, finish, t1, over_budget)
+    res, spent = run_to_end(b, n, chunk, steps, finish, t1, over_budget, label=f"warm batch to {levels[0]:g}")
     relax += spent
     for k, level in enumerate(levels):
         if k > 0:
             # Per-case finish times are those of the last level.
             set_threshold(b, n, level)
             finish[:] = np.nan
-            res, spent = run_to_end(b, n, chunk, steps, finish, t1)
+            res, spent = run_to_end(b, n, chunk, steps, finish, t1, label=f"warm batch to {level:g}")
             relax += spent
         level_steps.append(steps.copy())
         level_wall.append(relax)
