@@ -374,7 +374,8 @@ def mode_warmstart(args):
     levels = [float(v) for v in args.warm_levels.split(",")] if args.warm_levels else None
     warmstart.experiment(args.L, args.M, args.train, args.test, args.epochs, args.batch, args.lr, args.clip,
                          args.plain_sample, args.seed, args.out, warm_levels=levels,
-                         ref_threshold=args.ref_threshold, load=args.load)
+                         ref_threshold=args.ref_threshold, load=args.load, gen_batch=args.gen_batch,
+                         net_grid=tuple(int(v) for v in args.net_grid.split("x")) if args.net_grid else None)
 
 
 def mode_nn_train(args):
@@ -423,6 +424,9 @@ def main():
     ap.add_argument("--ref-threshold", type=float, default=None,
                     help="also build reference steady states converged to this level and report errors against them")
     ap.add_argument("--load", default="", help="a saved warmstart_net.npz (skips generation and training)")
+    ap.add_argument("--gen-batch", type=int, default=512, help="cases per batch of the steady-state generation")
+    ap.add_argument("--net-grid", default="", help="grid LxM of a loaded network when it differs from --L/--M "
+                    "(its prediction is interpolated onto the run's grid)")
     ap.add_argument("--H", type=int, default=96)
     ap.add_argument("--W", type=int, default=40)
     args = ap.parse_args()

@@ -208,6 +208,31 @@ def predict(net, norm, params, radii, L, M, batch=32):
     return out
 
 
+def resample_fields(f, L0, M0, L1, M1):
+    """Fields (n, L0, M0 + 1, 8) on one grid of the unit (z, frac) square
+    (cell l at z = (l - 1) / L0, node m at frac = (m - 1) / M0) bilinearly
+    interpolated onto another, constant beyond the source's last row."""
+    f = np.asarray(f, np.float64)
+    z0 = np.arange(L0) / L0
+    z1 = np.arange(L1) / L1
+    y0 = np.arange(M0 + 1) / M0
+    y1 = np.arange(M1 + 1) / M1
+    iz = np.clip(np.searchsorted(z0, z1, side="right") - 1, 0, L0 - 2)
+    wz = np.clip((z1 - z0[iz]) / (z0[iz + 1] - z0[iz]), 0.0, 1.0)
+    iy = np.clip(np.searchsorted(y0, y1, side="right") - 1, 0, M0 - 1)
+    wy = np.clip((y1 - y0[iy]) / (y0[iy + 1] - y0[iy]), 0.0, 1.0)
+    rows = f[:, iz] * (1.0 - wz)[None, :, None, None] + f[:, iz + 1] * wz[None, :, None, None]
+    return rows[:, :, iy] * (1.0 - wy)[None, None, :, None] + rows[:, :, iy + 1] * wy[None, None, :, None]
+
+
+def predict_on_grid(net, norm, params, L_net, M_net, L, M, batch=32):
+    """The network's prediction on its own grid (L_net x M_net), interpolated
+    onto an L x M grid of the same cases."""
+    radii = grid_radii(params, L_net, M_net)
+    f = predict(net, norm, params, radii, L_net, M_net, batch=batch)
+    return f if (L_net, M_net) == (L, M) else resample_fields(f, L_net, M_net, L, M)
+
+
 def warm_u0(fields, radii, L, M):
     rows, cols = L + 2, M + 3
     return [SG.u0_from_fields(fields[i], radii[i], rows, cols) for i in range(len(fields))]
@@ -361,6 +386,8 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     t0 = time.perf_counter()
     if pred is None:
         pred = predict(net, norm, params, radii, L, M)
+    elif callable(pred):
+        pred = pred()  # e.g. a prediction on a coarser grid, interpolated (inside the timing)
     infer = time.perf_counter() - t0
     b = mpd_amd.Batch(cfgs)
     rejected = [i for i in range(n) if not admissible(pred[i])]
@@ -486,15 +513,16 @@ def load_network(path, L, M, seed=2024):
 
 
 def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, clip=None, plain_sample=16,
-               seed=2024, out="", warm_levels=None, ref_threshold=None, load="", gen_batch=512):
+               seed=2024, out="", warm_levels=None, ref_threshold=None, load="", gen_batch=512, net_grid=None):
     """The whole experiment: train_n steady states generated, the network
     trained on them (or a saved one loaded), test_n held-out cases evaluated
     (evaluate); prints the result as JSON and, with `out`, writes it and the
     trained weights there."""
+    Ln, Mn_ = net_grid if net_grid else (L, M)
     if load:
-        net, norm, train_steps = load_network(load, L, M, seed)
-        result_extra = {"loaded_network": os.path.basename(load)}
-        progress(f"network loaded from {load}")
+        net, norm, train_steps = load_network(load, Ln, Mn_, seed)
+        result_extra = {"loaded_network": os.path.basename(load), "network_grid": [Ln, Mn_]}
+        progress(f"network loaded from {load} (grid {Ln}x{Mn_})")
     else:
         t0 = time.perf_counter()
         params, fields, steps, conv = generate(train_n, L, M, seed, batch=gen_batch)
@@ -534,8 +562,16 @@ def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, cli
                 json.dump(result_extra, f, indent=1)
         return result_extra
     test_params = sweep.sample_params(test_n, seed + 1)
-    result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=plain_sample,
-                            budget=step_budget(train_steps), warm_levels=warm_levels, ref_threshold=ref_threshold)
+    pred = None
+    budget = step_budget(train_steps)
+    if (Ln, Mn_) != (L, M):
+        # A network of a coarser grid: its prediction interpolated onto this
+        # grid; the step budget scales with the grid's cold step counts
+        # (not known here: none).
+        pred = lambda: predict_on_grid(net, norm, test_params, Ln, Mn_, L, M)  # noqa: E731
+        budget = None
+    result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=plain_sample, pred=pred,
+                            budget=budget, warm_levels=warm_levels, ref_threshold=ref_threshold)
     result.update(result_extra)
     result["n_params"] = net.n_params()
     print(json.dumps(result, indent=1), flush=True)

@@ -111,3 +111,28 @@ def test_padding_for_the_network_levels():
     assert np.array_equal(p[:, :5, 3:], np.repeat(a[:, :, 2:3], 5, axis=2))
     assert np.all(p[:, 5:, 3:] == a[:, 4:5, 2:3])
     assert SG.pad_grid(a, 5, 3) is a
+
+
+def test_resample_fields_bilinear_exact():
+    """A field bilinear in (z, frac) is reproduced by the interpolation onto a
+    finer grid wherever the target lies inside the source's z range (constant
+    beyond its last row); the same grid is the identity."""
+    L0, M0, L1, M1 = 10, 4, 20, 8
+    z0 = np.arange(L0) / L0
+    y0 = np.arange(M0 + 1) / M0
+    f = np.zeros((2, L0, M0 + 1, 8))
+    for k in range(8):
+        f[0, ..., k] = (1.0 + k) + 2.0 * z0[:, None] - 0.5 * y0[None, :] + 0.75 * z0[:, None] * y0[None, :]
+        f[1, ..., k] = -f[0, ..., k]
+    g = WS.resample_fields(f, L0, M0, L1, M1)
+    assert g.shape == (2, L1, M1 + 1, 8)
+    z1 = np.arange(L1) / L1
+    y1 = np.arange(M1 + 1) / M1
+    inside = z1 <= z0[-1]
+    for k in range(8):
+        want = (1.0 + k) + 2.0 * z1[:, None] - 0.5 * y1[None, :] + 0.75 * z1[:, None] * y1[None, :]
+        np.testing.assert_allclose(g[0, inside, :, k], want[inside], rtol=1e-13, atol=1e-13)
+        np.testing.assert_allclose(g[0, ~inside, :, k], np.broadcast_to(g[0, inside, :, k][-1], g[0, ~inside, :, k].shape),
+                                   rtol=1e-13)
+    np.testing.assert_allclose(g[1], -g[0], rtol=0, atol=0)
+    assert np.array_equal(WS.resample_fields(f, L0, M0, L0, M0), f)
