@@ -375,7 +375,8 @@ def mode_warmstart(args):
     warmstart.experiment(args.L, args.M, args.train, args.test, args.epochs, args.batch, args.lr, args.clip,
                          args.plain_sample, args.seed, args.out, warm_levels=levels,
                          ref_threshold=args.ref_threshold, load=args.load, gen_batch=args.gen_batch,
-                         net_grid=tuple(int(v) for v in args.net_grid.split("x")) if args.net_grid else None)
+                         net_grid=tuple(int(v) for v in args.net_grid.split("x")) if args.net_grid else None,
+                         ref_max_steps=args.ref_max_steps, case_max_steps=args.case_max_steps)
 
 
 def mode_nn_train(args):
@@ -425,6 +426,9 @@ def main():
                     help="also build reference steady states converged to this level and report errors against them")
     ap.add_argument("--load", default="", help="a saved warmstart_net.npz (skips generation and training)")
     ap.add_argument("--gen-batch", type=int, default=512, help="cases per batch of the steady-state generation")
+    ap.add_argument("--ref-max-steps", type=int, default=2000000,
+                    help="most steps a reference state runs beyond the sweep threshold")
+    ap.add_argument("--case-max-steps", type=int, default=2000000, help="time.max_steps of every evaluated case")
     ap.add_argument("--net-grid", default="", help="grid LxM of a loaded network when it differs from --L/--M "
                     "(its prediction is interpolated onto the run's grid)")
     ap.add_argument("--H", type=int, default=96)

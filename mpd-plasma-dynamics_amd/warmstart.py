@@ -344,7 +344,7 @@ def reference_states(cfgs, L, M, ref_threshold, max_steps, chunk=20000):
 
 def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_change=1e-6, plain_sample=None,
              chunk=1000, fallback=True, pred=None, budget=None, warm_levels=None, ref_threshold=None,
-             ref_max_steps=2000000):
+             ref_max_steps=2000000, max_steps=2000000):
     """Cold-started plain solves (each case alone, mpd_run) against the
     surrogate-warm-started batched sweep, both to the same threshold.  A
     warm case whose SolutionChange is still above stall_change after
@@ -359,7 +359,7 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     (reference_states) and report every state's per-field error against
     them."""
     n = len(params)
-    cfgs = [sweep.case_config(p, L, M, threshold=threshold, max_steps=2000000) for p in params]
+    cfgs = [sweep.case_config(p, L, M, threshold=threshold, max_steps=max_steps) for p in params]
     radii = grid_radii(params, L, M)
     levels = [threshold] + [v for v in (warm_levels or []) if v < threshold]
 
@@ -485,19 +485,36 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
                  f"({time.perf_counter() - t2:.1f}s, steps p50 {np.median(ref_steps):.0f})")
 
         def worst(states):
+            # Over the cases whose reference reached the level (a case that
+            # stagnates short of it has no converged reference to measure
+            # against); per-case worst field for the distribution.
             e = np.array([field_errors(states[i], ref_fields[i]) for i in range(n)])
-            return dict(zip(SG.OUT_FIELDS, e.max(axis=0).tolist())), float(e.max())
+            per_case = e.max(axis=1)
+            ok = e[reached] if reached.any() else e
+            return (dict(zip(SG.OUT_FIELDS, ok.max(axis=0).tolist())), float(ok.max()),
+                    {"p50": float(np.median(per_case[reached])) if reached.any() else None,
+                     "p90": float(np.percentile(per_case[reached], 90)) if reached.any() else None,
+                     "cases_le_1e-6": int(np.sum(per_case[reached] <= 1e-6)),
+                     "per_case_all": per_case.tolist()})
 
-        cold_ref, cold_max = worst(cold_fields)
+        cold_ref, cold_max, cold_dist = worst(cold_fields)
         out.update({"reference": {"threshold": ref_threshold, "reached": int(reached.sum()),
+                                  "reached_per_case": reached.astype(int).tolist(),
                                   "steps_p50": float(np.median(ref_steps)), "steps_max": int(ref_steps.max()),
                                   "max_steps_beyond_threshold": ref_max_steps},
-                    "max_field_err_cold_vs_reference": cold_ref, "max_err_cold_vs_reference": cold_max})
+                    "max_field_err_cold_vs_reference": cold_ref, "max_err_cold_vs_reference": cold_max,
+                    "err_cold_vs_reference_dist": cold_dist})
         for k, lv in enumerate(levels):
-            e, mx = worst(level_fields[k])
-            out["warm_levels"][k].update({"max_field_err_vs_reference": e, "max_err_vs_reference": mx})
+            e, mx, dist = worst(level_fields[k])
+            out["warm_levels"][k].update({"max_field_err_vs_reference": e, "max_err_vs_reference": mx,
+                                          "err_vs_reference_dist": dist})
         out["max_field_err_warm_vs_reference"] = out["warm_levels"][-1]["max_field_err_vs_reference"]
         out["max_err_warm_vs_reference"] = out["warm_levels"][-1]["max_err_vs_reference"]
+    out["per_case"] = {"cold_steps": [int(r["steps"]) for r in cold_res], "warm_steps": steps.astype(int).tolist(),
+                       "warm_converged": [int(r["converged"]) for r in res],
+                       "restarted_cold": sorted(int(i) for i in restarted),
+                       "cold_finish_s": np.nan_to_num(cold_finish, nan=-1.0).tolist(),
+                       "warm_finish_s": np.nan_to_num(finish, nan=-1.0).tolist()}
     return out, warm_fields, cold_fields
 
 
@@ -513,7 +530,8 @@ def load_network(path, L, M, seed=2024):
 
 
 def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, clip=None, plain_sample=16,
-               seed=2024, out="", warm_levels=None, ref_threshold=None, load="", gen_batch=512, net_grid=None):
+               seed=2024, out="", warm_levels=None, ref_threshold=None, load="", gen_batch=512, net_grid=None,
+               ref_max_steps=2000000, case_max_steps=2000000):
     """The whole experiment: train_n steady states generated, the network
     trained on them (or a saved one loaded), test_n held-out cases evaluated
     (evaluate); prints the result as JSON and, with `out`, writes it and the
@@ -571,7 +589,8 @@ def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, cli
         pred = lambda: predict_on_grid(net, norm, test_params, Ln, Mn_, L, M)  # noqa: E731
         budget = None
     result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=plain_sample, pred=pred,
-                            budget=budget, warm_levels=warm_levels, ref_threshold=ref_threshold)
+                            budget=budget, warm_levels=warm_levels, ref_threshold=ref_threshold,
+                            ref_max_steps=ref_max_steps, max_steps=case_max_steps)
     result.update(result_extra)
     result["n_params"] = net.n_params()
     print(json.dumps(result, indent=1), flush=True)
