@@ -109,9 +109,11 @@ inline void build_bc_image(const MpdBcProgram& p, uint64_t* out) {
 }
 
 // Per-lane work area of the boundary kernel, in slots of one double: the
-// expression symbol table, the chain state and its neighbour state, then the
-// evaluation stack, as deep as the program's deepest expression needs.
-enum { WA_SYM = 0, WA_CUR = WA_SYM + MPD_BS_COUNT, WA_NBR = WA_CUR + 10, WA_STK = WA_NBR + 10 };
+// expression symbol table, then the evaluation stack, as deep as the
+// program's deepest expression needs.
+enum { WA_SYM = 0, WA_STK = WA_SYM + MPD_BS_COUNT };
+// Expressions at most this deep evaluate on a stack held in registers.
+constexpr int kRegStack = 4;
 // Boundary-kernel lanes of the ring of an L x Mn grid.  Compact (the
 // batched kernel, where many instances' workgroups must fit at once): row 1,
 // row L, column 1 (l = 2..L-1), column Mn, back to back.  Grouped (a single

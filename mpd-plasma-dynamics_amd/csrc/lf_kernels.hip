@@ -568,11 +568,10 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled_batch(const StepArgs*
 // ---------------------------------------------------- boundary conditions
 
 // Per-thread work area in LDS for the boundary evaluation: the expression
-// symbol table and evaluation stack, and the state and neighbour state of
-// the face chain being applied -- everything the condition loop indexes at
-// run time (in registers these would be private arrays in scratch memory).
-// Laid out [slot][lane] for the 64-thread workgroups of lf_bc_ring, so one
-// slot of a wave is 64 consecutive doubles.
+// symbol table (read at run-time indices) and, for expressions deeper than
+// the register stack, the evaluation stack.  Laid out [slot][lane] for the
+// 64-thread workgroups of lf_bc_ring, so one slot of a wave is 64
+// consecutive doubles.
 #define BC_THREADS 64
 static_assert(BC_THREADS == 64, "ring_lanes groups one kind of cell per 64-lane boundary workgroup");
 struct WorkArea {
@@ -581,17 +580,6 @@ struct WorkArea {
     __device__ __forceinline__ double& at(int i) const { return base[i * BC_THREADS + lane]; }
     __device__ __forceinline__ double& sym(int i) const { return at(WA_SYM + i); }
     __device__ __forceinline__ double& stk(int i) const { return at(WA_STK + i); }
-    __device__ __forceinline__ double& cur(int i) const { return at(WA_CUR + i); }
-    __device__ __forceinline__ double& nbr(int i) const { return at(WA_NBR + i); }
-    // Copies between a register array of 10 physical values and an area.
-    __device__ __forceinline__ void put(int area, const double* v) const {
-#pragma unroll
-        for (int k = 0; k < 10; ++k) at(area + k) = v[k];
-    }
-    __device__ __forceinline__ void get(int area, double* v) const {
-#pragma unroll
-        for (int k = 0; k < 10; ++k) v[k] = at(area + k);
-    }
 };
 
 // The staged image a boundary kernel's StepArgs points to.
@@ -621,8 +609,10 @@ __device__ __noinline__ double bc_fn2(int op, double x, double y) {
 }
 
 // Evaluate one compiled expression (the exprtk call sites of
-// src/bcs/per_field_bc.cpp:323-358) on the symbol values in the work area.
-__device__ __forceinline__ double eval_expr(const BcImage* bc, int off, int len, const WorkArea& w) {
+// src/bcs/per_field_bc.cpp:323-358) on the symbol values in the work area,
+// with the evaluation stack in the work area too: programs deeper than the
+// register stack (out of line: one copy serves every call site).
+__device__ __noinline__ double eval_expr_lds(const BcImage* bc, int off, int len, WorkArea w) {
     int sp = 0;
     for (int i = 0; i < len; ++i) {
         const MpdInstr in = bc->code(off + i);
@@ -671,6 +661,81 @@ __device__ __forceinline__ double eval_expr(const BcImage* bc, int off, int len,
     }
     return w.stk(0);
 }
+
+// Push and pop of the register stack s[0..kRegStack) (s[0] the top).
+__device__ __forceinline__ void rs_push(double* s, double v) {
+#pragma unroll
+    for (int k = kRegStack - 1; k > 0; --k) s[k] = s[k - 1];
+    s[0] = v;
+}
+// Replace the top n + 1 entries by r.
+__device__ __forceinline__ void rs_reduce(double* s, int n, double r) {
+    s[0] = r;
+#pragma unroll
+    for (int k = 1; k + n < kRegStack; ++k) s[k] = s[k + n];
+}
+__device__ __forceinline__ double rs_pick(bool is_min, double acc, double v) {
+    return is_min ? ((v < acc) ? v : acc) : ((acc < v) ? v : acc);
+}
+
+// The same evaluation with the stack in registers, for programs whose
+// deepest expression needs at most kRegStack entries.  The same operations
+// in the same order as eval_expr_lds, so both give the same bits; the next
+// instruction is read while the current one executes.
+__device__ __forceinline__ double eval_expr_regs(const BcImage* bc, int off, int len, const WorkArea& w) {
+    double s[kRegStack];
+#pragma unroll
+    for (int k = 0; k < kRegStack; ++k) s[k] = 0.0;
+    if (len <= 0) return 0.0;
+    MpdInstr next = bc->code(off);
+    for (int i = 0; i < len; ++i) {
+        const MpdInstr in = next;
+        if (i + 1 < len) next = bc->code(off + i + 1);
+        switch (in.op) {
+            case MPD_OP_CONST: rs_push(s, in.val); break;
+            case MPD_OP_SYM: rs_push(s, w.sym(in.arg)); break;
+            case MPD_OP_NEG: s[0] = -s[0]; break;
+            case MPD_OP_ADD: rs_reduce(s, 1, s[1] + s[0]); break;
+            case MPD_OP_SUB: rs_reduce(s, 1, s[1] - s[0]); break;
+            case MPD_OP_MUL: rs_reduce(s, 1, s[1] * s[0]); break;
+            case MPD_OP_DIV: rs_reduce(s, 1, s[1] / s[0]); break;
+            case MPD_OP_MOD:
+            case MPD_OP_POW:
+            case MPD_OP_ATAN2: rs_reduce(s, 1, bc_fn2(in.op, s[1], s[0])); break;
+            case MPD_OP_F1: {
+                const double x = s[0];
+                double y;
+                if (in.arg == MPD_FN_SQRT) y = sqrt(x);
+                else if (in.arg == MPD_FN_ABS) y = fabs(x);
+                else if (in.arg == MPD_FN_SIGN) y = (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0);
+                else y = bc_fn1(in.arg, x);
+                s[0] = y;
+                break;
+            }
+            case MPD_OP_CLAMP: {
+                const double lo = s[2], x = s[1], hi = s[0];
+                rs_reduce(s, 2, (x < lo) ? lo : ((x > hi) ? hi : x));
+                break;
+            }
+            case MPD_OP_MIN:
+            case MPD_OP_MAX: {
+                // Arguments s[n-1] (first) .. s[0] (last), folded first to last.
+                const bool mn = in.op == MPD_OP_MIN;
+                if (in.arg == 2) {
+                    rs_reduce(s, 1, rs_pick(mn, s[1], s[0]));
+                } else if (in.arg == 3) {
+                    rs_reduce(s, 2, rs_pick(mn, rs_pick(mn, s[2], s[1]), s[0]));
+                } else if (in.arg == 4) {
+                    rs_reduce(s, 3, rs_pick(mn, rs_pick(mn, rs_pick(mn, s[3], s[2]), s[1]), s[0]));
+                }
+                break;
+            }
+            default: break;
+        }
+    }
+    return s[0];
+}
+static_assert(kRegStack == 4, "eval_expr_regs folds MIN/MAX of up to four arguments");
 
 // Physical slot of each condition (evaluation order rho, v_z, v_r, v_phi,
 // H_z, H_r, H_phi, e) and of each neighbour symbol (SetNeighbors order rho,
@@ -732,50 +797,60 @@ __device__ __forceinline__ void load_ring_geo(const StepArgs& a, int l, int m, R
 }
 
 // Apply the segments [0, seg_end) of face f that cover cell (l, m) to the
-// physical state in the work area's `cur` slots, with the neighbour state in
-// its `nbr` slots: PerFieldBC::Apply's per-cell physical part
-// (src/bcs/per_field_bc.cpp:297-360, 391-446).  Returns the index of the last
-// segment applied, or -1.  The condition loop stays rolled: one copy of the
-// interpreter per call site.
+// physical state cur[10], with the neighbour state nbr[10] (both indexed by
+// PH_*): PerFieldBC::Apply's per-cell physical part
+// (src/bcs/per_field_bc.cpp:297-360, 391-446).  Returns the index of the
+// last segment applied, or -1.
+// Per segment the Neumann and Dirichlet conditions resolve first, in
+// registers: they read only the neighbour state and constants.  The
+// expressions then run in field order; an expression reads only the fields
+// evaluated before it (forward references are rejected when the program is
+// built), so each sees the values the serial order rho, v_z, ..., e gives it.
 __device__ __forceinline__ int face_chain(const StepArgs& a, const WorkArea& w, int f, int l, int m, int seg_end,
-                                          const ChainGeo& cg) {
+                                          const ChainGeo& cg, double* cur, const double* nbr) {
     const BcImage* bc = img(a);
     const bool is_l = (f == MPD_L_LO || f == MPD_L_HI);
     const int i = is_l ? m : l;
-    const double g_dr = cg.dr, g_r = cg.r, g_rz = cg.rz;
-    const double g_rnb = cg.rnb, g_rznb = cg.rznb;
+    const bool regs = a.bc_area - WA_STK <= kRegStack;
     int last = -1;
     for (int s = 0; s < seg_end; ++s) {
         const MpdSeg& g = bc->seg(f, s);
         if (!seg_covers(g, i)) continue;
         last = s;
-        if (g.has_expr) {
-            w.sym(MPD_BS_GAMMA) = a.gamma;
-            w.sym(MPD_BS_BETA) = a.beta;
-            w.sym(MPD_BS_HZ0) = a.H_z0;
-            w.sym(MPD_BS_R0) = a.r_0;
-            w.sym(MPD_BS_DZ) = a.dz;
-            w.sym(MPD_BS_DR) = g_dr;
-            w.sym(MPD_BS_R) = g_r;
-            w.sym(MPD_BS_RZ) = g_rz;
-            w.sym(MPD_BS_Z) = cg.z;
-            w.sym(MPD_BS_RNB) = g_rnb;
-            w.sym(MPD_BS_RZNB) = g_rznb;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) w.sym(MPD_BS_RHO_NB + k) = w.nbr(nb_slot(k));
+        for (int k = 0; k < 8; ++k) {
+            const MpdCond& c = g.c[k];
+            const int slot = cond_slot(k);
+            if (c.type == MPD_NEUMANN) cur[slot] = nbr[slot] + c.value;
+            else if (c.type == MPD_DIRICHLET) cur[slot] = c.value;
+        }
+        if (!g.has_expr) continue;
+        w.sym(MPD_BS_GAMMA) = a.gamma;
+        w.sym(MPD_BS_BETA) = a.beta;
+        w.sym(MPD_BS_HZ0) = a.H_z0;
+        w.sym(MPD_BS_R0) = a.r_0;
+        w.sym(MPD_BS_DZ) = a.dz;
+        w.sym(MPD_BS_DR) = cg.dr;
+        w.sym(MPD_BS_R) = cg.r;
+        w.sym(MPD_BS_RZ) = cg.rz;
+        w.sym(MPD_BS_Z) = cg.z;
+        w.sym(MPD_BS_RNB) = cg.rnb;
+        w.sym(MPD_BS_RZNB) = cg.rznb;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            w.sym(MPD_BS_RHO_NB + k) = nbr[nb_slot(k)];
+            w.sym(MPD_BS_RHO + k) = cur[cond_slot(k)];
         }
 #pragma unroll 1
         for (int k = 0; k < 8; ++k) {
-            const int slot = cond_slot(k);
             const MpdCond& c = g.c[k];
-            double v;
-            if (c.type == MPD_NEUMANN) v = w.nbr(slot) + c.value;
-            else if (c.type == MPD_DIRICHLET) v = c.value;
-            else if (c.type == MPD_EXPRESSION) v = eval_expr(bc, c.code_off, c.code_len, w);
-            else v = w.cur(slot);
-            w.cur(slot) = v;
-            if (g.has_expr) w.sym(MPD_BS_RHO + k) = v;
+            if (c.type != MPD_EXPRESSION) continue;
+            w.sym(MPD_BS_RHO + k) = regs ? eval_expr_regs(bc, c.code_off, c.code_len, w)
+                                         : eval_expr_lds(bc, c.code_off, c.code_len, w);
         }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (g.c[k].type == MPD_EXPRESSION) cur[cond_slot(k)] = w.sym(MPD_BS_RHO + k);
     }
     return last;
 }
@@ -831,10 +906,7 @@ __device__ __forceinline__ void mlo_finish(const StepArgs& a, const WorkArea& w,
     if (p.cf >= 0) {
         double nb[10];
         src_phys(p.n, p.vn, cell_r(a, p.nl, p.nm), a.gm1, nb);
-        w.put(WA_CUR, ph);
-        w.put(WA_NBR, nb);
-        face_chain(a, w, p.cf, p.l, 1, p.ce, chain_geo_rows(a, p.cf, p.l, 1));
-        w.get(WA_CUR, ph);
+        face_chain(a, w, p.cf, p.l, 1, p.ce, chain_geo_rows(a, p.cf, p.l, 1), ph, nb);
     }
 }
 
@@ -924,8 +996,6 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
     if (pf >= 0) {
         double nb2[10];
         src_phys(sq, vq, G.v[RG_R_DIAG], a.gm1, nb2);
-        w.put(WA_CUR, nb);
-        w.put(WA_NBR, nb2);
         // M_HI's corner (1, Mn): L_LO's chain at (1, Mn - 1), the column
         // neighbour; L_HI's corners: M_LO's / M_HI's at (L - 1, m), the row
         // neighbour.  Either way the chain's own neighbour is the diagonal.
@@ -933,14 +1003,10 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
             ? ChainGeo{G.v[RG_DR], G.v[RG_R_COL], G.v[RG_RZ_COL], G.v[RG_Z], G.v[RG_R_DIAG], G.v[RG_RZ_DIAG]}
             : ChainGeo{G.v[RG_DR_ROW], G.v[RG_R_ROW], G.v[RG_RZ_ROW], G.v[RG_Z_ROW], G.v[RG_R_DIAG],
                        G.v[RG_RZ_DIAG]};
-        face_chain(a, w, pf, pl, pm, img(a)->nseg[pf], pg);
-        w.get(WA_CUR, nb);
+        face_chain(a, w, pf, pl, pm, img(a)->nseg[pf], pg, nb, nb2);
     }
-    w.put(WA_CUR, st);
-    w.put(WA_NBR, nb);
     const int last = face_chain(a, w, f, l, m, img(a)->nseg[f],
-                                ChainGeo{G.v[RG_DR], r, G.v[RG_RZ], G.v[RG_Z], r_nb, rz_nb});
-    w.get(WA_CUR, st);
+                                ChainGeo{G.v[RG_DR], r, G.v[RG_RZ], G.v[RG_Z], r_nb, rz_nb}, st, nb);
     rebuild_u(st, r, u);
     if (f == MPD_M_LO && last >= 0 && img(a)->seg(f, last).has_axis) {
         const MpdSeg& g = img(a)->seg(f, last);
