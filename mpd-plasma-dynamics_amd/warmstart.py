@@ -277,10 +277,12 @@ def step_budget(train_steps, factor=1.5):
     return int(factor * train_steps.max()) if train_steps.size else None
 
 
-def set_threshold(b, n, level):
-    """A new convergence stop level for every instance of batch b."""
+def set_threshold(b, n, level, skip=()):
+    """A new convergence stop level for every instance of batch b but those
+    in `skip`."""
     for i in range(n):
-        b.instance(i).set_option("convergence_threshold", level)
+        if i not in skip:
+            b.instance(i).set_option("convergence_threshold", level)
 
 
 def run_to_end(b, n, chunk, steps, finish=None, t_start=None, on_chunk=None, label="batch"):
@@ -424,6 +426,7 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
             for i, r in enumerate(res):
                 if not r["done"] and i not in restarted and steps[i] > budget:
                     b.instance(i).cold_restart(cfgs[i])
+                    b.instance(i).set_option("convergence_threshold", threshold)
                     restarted.add(i)
                     late.append(i)
                     out.append(i)
@@ -434,10 +437,12 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     relax += spent
     for k, level in enumerate(levels):
         if k > 0:
-            # Per-case finish times are those of the last level.
-            set_threshold(b, n, level)
+            # Per-case finish times are those of the last level.  A case
+            # restarted cold keeps the sweep threshold (a cold solve's own
+            # stop rule); the step budget still applies to the warm ones.
+            set_threshold(b, n, level, skip=restarted)
             finish[:] = np.nan
-            res, spent = run_to_end(b, n, chunk, steps, finish, t1, label=f"warm batch to {level:g}")
+            res, spent = run_to_end(b, n, chunk, steps, finish, t1, over_budget, label=f"warm batch to {level:g}")
             relax += spent
         level_steps.append(steps.copy())
         level_wall.append(relax)
