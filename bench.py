@@ -376,7 +376,8 @@ def mode_warmstart(args):
                          args.plain_sample, args.seed, args.out, warm_levels=levels,
                          ref_threshold=args.ref_threshold, load=args.load, gen_batch=args.gen_batch,
                          net_grid=tuple(int(v) for v in args.net_grid.split("x")) if args.net_grid else None,
-                         ref_max_steps=args.ref_max_steps, case_max_steps=args.case_max_steps)
+                         ref_max_steps=args.ref_max_steps, case_max_steps=args.case_max_steps,
+                         staged=args.warm_staged)
 
 
 def mode_nn_train(args):
@@ -422,6 +423,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default="")
     ap.add_argument("--warm-levels", default="", help="tighter warm-start stop levels, e.g. 1e-9,1e-10")
+    ap.add_argument("--warm-staged", action="store_true",
+                    help="run the whole warm batch to each level in turn and compare the states of every level "
+                         "(default: each warm case runs straight to the last level)")
     ap.add_argument("--ref-threshold", type=float, default=None,
                     help="also build reference steady states converged to this level and report errors against them")
     ap.add_argument("--load", default="", help="a saved warmstart_net.npz (skips generation and training)")

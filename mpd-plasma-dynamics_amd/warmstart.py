@@ -346,7 +346,7 @@ def reference_states(cfgs, L, M, ref_threshold, max_steps, chunk=20000):
 
 def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_change=1e-6, plain_sample=None,
              chunk=1000, fallback=True, pred=None, budget=None, warm_levels=None, ref_threshold=None,
-             ref_max_steps=2000000, max_steps=2000000):
+             ref_max_steps=2000000, max_steps=2000000, staged=False):
     """Cold-started plain solves (each case alone, mpd_run) against the
     surrogate-warm-started batched sweep, both to the same threshold.  A
     warm case whose SolutionChange is still above stall_change after
@@ -354,9 +354,14 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     running after `budget` steps (step_budget; None: no budget).
     plain_sample: time only that many plain solves and scale to all cases
     (stated in the result).
-    warm_levels: tighter convergence levels the warm batch continues to after
-    `threshold` (the warm start's stop rule is the last one; the wall time and
-    steps reported are to it, and the states at every level are compared).
+    warm_levels: tighter convergence levels for the warm-started cases (the
+    warm start's stop rule is the last one; the wall time and steps reported
+    are to it).  staged=False: every warm case runs straight to the last
+    level (the timing protocol: each case stops when it reaches it).
+    staged=True: the whole batch runs to `threshold`, then to each level in
+    turn, and the states at every level are compared (the accuracy study; a
+    case's finish time then waits for the slowest case of the level before).
+    Cases that start or restart cold always stop at `threshold`.
     ref_threshold: also build tightly converged FP64 steady states
     (reference_states) and report every state's per-field error against
     them."""
@@ -364,6 +369,8 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     cfgs = [sweep.case_config(p, L, M, threshold=threshold, max_steps=max_steps) for p in params]
     radii = grid_radii(params, L, M)
     levels = [threshold] + [v for v in (warm_levels or []) if v < threshold]
+    if not staged:
+        levels = levels[-1:]
 
     # Plain cold solves, one case at a time (the reference's way: one run per case).
     m = n if plain_sample is None else min(n, plain_sample)
@@ -396,6 +403,8 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     for i, u in enumerate(warm_u0(pred, radii, L, M)):
         if i not in rejected:
             b.warm_start(i, u)
+    if levels[0] != threshold:
+        set_threshold(b, n, levels[0], skip=set(rejected))
     b.autotune(5)
     t1 = time.perf_counter()
     # Stall check: after stall_steps, a case still running with its
@@ -415,6 +424,7 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
              stalled(r["done"], r["converged"], r["change"], mid[i], stall_change)]
     for i in stall:
         b.instance(i).cold_restart(cfgs[i])
+        b.instance(i).set_option("convergence_threshold", threshold)
         finish[i] = np.nan
     restarted = set(stall) | set(rejected)
     late = []
@@ -536,7 +546,7 @@ def load_network(path, L, M, seed=2024):
 
 def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, clip=None, plain_sample=16,
                seed=2024, out="", warm_levels=None, ref_threshold=None, load="", gen_batch=512, net_grid=None,
-               ref_max_steps=2000000, case_max_steps=2000000):
+               ref_max_steps=2000000, case_max_steps=2000000, staged=False):
     """The whole experiment: train_n steady states generated, the network
     trained on them (or a saved one loaded), test_n held-out cases evaluated
     (evaluate); prints the result as JSON and, with `out`, writes it and the
@@ -595,7 +605,7 @@ def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, cli
         budget = None
     result, _, _ = evaluate(net, norm, test_params, L, M, plain_sample=plain_sample, pred=pred,
                             budget=budget, warm_levels=warm_levels, ref_threshold=ref_threshold,
-                            ref_max_steps=ref_max_steps, max_steps=case_max_steps)
+                            ref_max_steps=ref_max_steps, max_steps=case_max_steps, staged=staged)
     result.update(result_extra)
     result["n_params"] = net.n_params()
     print(json.dumps(result, indent=1), flush=True)

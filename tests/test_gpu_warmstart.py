@@ -74,7 +74,8 @@ def test_trained_warm_start_saves_steps(ws, trained):
     import sweep
     net, norm = trained
     params = sweep.sample_params(8, 77)
-    r, warm, cold = ws.evaluate(net, norm, params, L, M, plain_sample=2, warm_levels=[1e-9], ref_threshold=1e-12)
+    r, warm, cold = ws.evaluate(net, norm, params, L, M, plain_sample=2, warm_levels=[1e-9], ref_threshold=1e-12,
+                                staged=True)
     print({k: r[k] for k in ("cold_steps_p50", "warm_steps_p50", "steps_ratio_cold_over_warm", "fallbacks",
                              "speedup_vs_cold_batch", "speedup_vs_plain_cold")}, flush=True)
     print("levels:", r["warm_levels"], flush=True)
