@@ -356,10 +356,13 @@ def mode_sweep_probe(args):
     with mpd_amd.Batch(sweep.probe_configs(args.probe)) as b:
         b.run(args.warmup)
         print(b.autotune(5), flush=True)
+        if args.persistent:
+            b.set_option("persistent", 1)
         t0 = time.perf_counter()
         b.run(args.steps)
         el = time.perf_counter() - t0
-    print(f"{args.probe}: {args.steps} steps {el:.3f} s = {el / args.steps * 1e6:.1f} us/step", flush=True)
+    mode = "persistent launches" if args.persistent else "graphs of per-phase launches"
+    print(f"{args.probe} ({mode}): {args.steps} steps {el:.3f} s = {el / args.steps * 1e6:.1f} us/step", flush=True)
 
 
 def mode_sweep_converge(args):
@@ -412,6 +415,8 @@ def main():
     ap.add_argument("--no-warmstart", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--probe", choices=["small", "uneven"], default="small")
+    ap.add_argument("--persistent", action="store_true",
+                    help="sweep-probe: run the batch as persistent launches (one workgroup per instance)")
     ap.add_argument("--sizes", default="48,96,144,192")
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=64)

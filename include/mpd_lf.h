@@ -204,9 +204,15 @@ int mpd_batch_size(const mpd_batch* b, int* n);
 int mpd_batch_instance(mpd_batch* b, int i, mpd_solver** out);
 
 /* Options: "steps_per_graph" (0: the smallest convergence check_frequency of
- * the running instances), "grid" (stencil workgroups; 0: one per work entry,
- * fewer: a persistent grid striding over the list), "tile_rows" (all
- * instances). */
+ * the running instances; also the steps per persistent launch), "grid"
+ * (stencil workgroups; 0: one per work entry, fewer: a persistent grid
+ * striding over the list), "tile_rows" (all instances), "persistent" (1: each
+ * run of K steps is one launch in which one workgroup per instance runs the
+ * instance's steps -- central update, boundary, step end, convergence check --
+ * separated by workgroup barriers; for sweeps of small instances, whose steps
+ * are otherwise bound by the latency of three launches each; 0, the default:
+ * a HIP graph of per-phase launches over all instances).  Results are the
+ * same bits either way. */
 int mpd_batch_set_option(mpd_batch* b, const char* key, double value);
 
 /* Autotuner: times the batched stencil launch on the current states (reps

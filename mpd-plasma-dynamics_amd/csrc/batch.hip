@@ -50,6 +50,8 @@ struct mpd_batch {
     mpd::BatchAux* d_aux = nullptr;
     mpd::BatchBlock* d_work = nullptr;
     size_t work_cap = 0;
+    mpd::BatchBlock* d_pwork = nullptr;  // work list of the persistent launch
+    size_t pwork_cap = 0;
     Pinned<int> h_icur;
     Pinned<MpdLoopState> h_st;
 
@@ -57,6 +59,9 @@ struct mpd_batch {
     std::vector<int> H;        // tile rows per instance
     int grid = 0;              // stencil workgroups (0: one per work entry)
     int steps_per_graph = 0;   // 0: chosen per run
+    // 1: the steps run as one persistent launch per K steps, one workgroup
+    // per instance (lf_batch_persist) instead of a graph of per-phase launches.
+    int persist = 0;
     std::string tuning;        // last autotuner report
 
     // Work lists of the captured graph: stencil entries, boundary entries,
@@ -74,7 +79,7 @@ struct mpd_batch {
     ~mpd_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
         inst.clear();  // before the stream and the loop states they borrow
-        void* bufs[] = {st, d_args, d_icur, d_aux, d_work};
+        void* bufs[] = {st, d_args, d_icur, d_aux, d_work, d_pwork};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (stream) (void)hipStreamDestroy(stream);
@@ -218,6 +223,38 @@ struct mpd_batch {
             HIPCHK(hipMemcpyAsync(d_work, w.data(), w.size() * sizeof(mpd::BatchBlock), hipMemcpyHostToDevice,
                                   stream));
         HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // K steps of every active instance in one launch of the persistent
+    // kernel: one work entry {instance, 0, tile count, tile rows, strips} per
+    // active instance, its program image and work areas in LDS.
+    void LaunchPersist(const std::vector<char>& active, int K) {
+        std::vector<mpd::BatchBlock> w;
+        int img = 0, area = 0;
+        for (int i = 0; i < n(); ++i) {
+            if (!active[i]) continue;
+            const mpd_solver& s = *inst[i];
+            w.push_back({i, 0, mpd::stencil_tiles(s.grid.L, s.grid.Mn, H[i]), H[i], mpd::stencil_strips(s.grid.Mn)});
+            img = std::max(img, s.bc_words);
+            area = std::max(area, s.bc_area);
+        }
+        if (w.empty()) return;
+        int dev = 0, max_lds = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t lds = mpd::persist_lds_bytes(img, area);
+        if (lds + 8192 > (size_t)max_lds)
+            throw ArgError("persistent batch: the boundary programs need " + std::to_string(lds) +
+                           " bytes of LDS per workgroup, more than the device's " + std::to_string(max_lds));
+        if (w.size() > pwork_cap) {
+            if (d_pwork) HIPCHK(hipFree(d_pwork));
+            d_pwork = nullptr;
+            pwork_cap = std::max(w.size(), (size_t)256);
+            HIPCHK(hipMalloc((void**)&d_pwork, pwork_cap * sizeof(mpd::BatchBlock)));
+        }
+        HIPCHK(hipMemcpyAsync(d_pwork, w.data(), w.size() * sizeof(mpd::BatchBlock), hipMemcpyHostToDevice, stream));
+        HIPCHK(mpd::launch_batch_persist(d_args, d_icur, d_aux, d_pwork, (int)w.size(), K, fast_div, buf, img, area,
+                                         stream));
     }
 
     void EnqueueStep(int flip, bool check) {
@@ -366,11 +403,17 @@ struct mpd_batch {
             for (int i = 0; i < N; ++i) any = any || active[i];
             if (!any) break;
             const int K = ChooseK(active);
-            const std::vector<char> due = DueSteps(active, K);
-            if (!graph || graph_K != K || graph_active != active || graph_due != due) Capture(active, due, K);
-            for (int i = 0; i < N; ++i) h_icur.p[i] = inst[i]->cur;
-            HIPCHK(hipMemcpyAsync(d_icur, h_icur.p, sizeof(int) * N, hipMemcpyHostToDevice, stream));
-            HIPCHK(hipGraphLaunch(graph, stream));
+            if (persist) {
+                for (int i = 0; i < N; ++i) h_icur.p[i] = inst[i]->cur;
+                HIPCHK(hipMemcpyAsync(d_icur, h_icur.p, sizeof(int) * N, hipMemcpyHostToDevice, stream));
+                LaunchPersist(active, K);
+            } else {
+                const std::vector<char> due = DueSteps(active, K);
+                if (!graph || graph_K != K || graph_active != active || graph_due != due) Capture(active, due, K);
+                for (int i = 0; i < N; ++i) h_icur.p[i] = inst[i]->cur;
+                HIPCHK(hipMemcpyAsync(d_icur, h_icur.p, sizeof(int) * N, hipMemcpyHostToDevice, stream));
+                HIPCHK(hipGraphLaunch(graph, stream));
+            }
             ReadLoops();
             bool rescans = false;
             for (int i = 0; i < N; ++i) {
@@ -566,6 +609,9 @@ int mpd_batch_set_option(mpd_batch* b, const char* key, double value) {
             const int h = (int)value;
             if (h < 1) throw ArgError("tile_rows must be >= 1");
             for (int i = 0; i < b->n(); ++i) b->H[i] = h;
+        } else if (k == "persistent") {
+            if (value != 0.0 && value != 1.0) throw ArgError("persistent must be 0 or 1");
+            b->persist = (int)value;
         } else {
             throw ArgError("unknown batch option '" + k + "'");
         }

@@ -1102,7 +1102,7 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt, i
                                               const uint64_t* prog, double* area, bool finish, const RingLane& rl) {
     StepArgs a = a_in;
     a.bc = prog;  // the staged image, read through img(a)
-    const WorkArea w{area, (int)threadIdx.x};
+    const WorkArea w{area, (int)(threadIdx.x & (BC_THREADS - 1))};
     const int L = a.L, Mn = a.Mn;
     uint64_t key = 0;
     bool rbad = false;
@@ -1217,12 +1217,9 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring_batch(const StepArgs* _
 // exact-rescan test and the dt bookkeeping of lf_bc_ring's last block, one
 // thread per instance.  The kernel boundary orders it after all of the step's
 // atomics, so no per-workgroup ticket or device-scope fence is needed.
-__global__ void __launch_bounds__(64) lf_finish_batch(const StepArgs* __restrict__ args, const int* __restrict__ icur,
-                                                      int n, int flip) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    const StepArgs a = args[2 * i + (icur[i] ^ flip)];
-    if (loop_idle(a)) return;
+// The end of one instance's step once all its cells are written: the
+// exact-rescan test, then the dt bookkeeping (finalize_step).
+__device__ __forceinline__ void finish_instance(const StepArgs& a) {
     const double T = speed_T(a, 1);
     const uint64_t bits = __hip_atomic_load(&a.st->speed_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const double mc = __longlong_as_double((long long)bits);
@@ -1232,6 +1229,15 @@ __global__ void __launch_bounds__(64) lf_finish_batch(const StepArgs* __restrict
     } else {
         finalize_step(a);
     }
+}
+
+__global__ void __launch_bounds__(64) lf_finish_batch(const StepArgs* __restrict__ args, const int* __restrict__ icur,
+                                                      int n, int flip) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const StepArgs a = args[2 * i + (icur[i] ^ flip)];
+    if (loop_idle(a)) return;
+    finish_instance(a);
 }
 
 // Exact max wave speed over the whole new state, run when the candidate max
@@ -1273,8 +1279,13 @@ __global__ void __launch_bounds__(256) lf_speed_fallback(StepArgs a) {
 // In the loop (`loop`), it runs only on a step that is a multiple of
 // check_frequency and was not checked yet, and sets converged/done
 // (src/main.cpp:233-247).
-__device__ __forceinline__ void residual_block(const StepArgs& a, double* prev, double* partial, int loop,
-                                               int check_frequency, double threshold, int save, int bid, int nblk) {
+
+// The sums of one 256-thread block's share of SolutionChange (and the
+// snapshot update when `save`): block `bid` of `nblk` strides over the
+// interior cells; the squares are summed per thread, then over the block by a
+// fixed tree.  The block's two sums are returned in thread 0.
+__device__ __forceinline__ void residual_sums(const StepArgs& a, double* prev, int save, int bid, int nblk,
+                                              double& bd, double& bc) {
     const int L = a.L, Mn = a.Mn;
     const int64_t n = (int64_t)L * Mn;
     double sd = 0.0, sc = 0.0;
@@ -1296,6 +1307,7 @@ __device__ __forceinline__ void residual_block(const StepArgs& a, double* prev, 
         sc += csum;
     }
     __shared__ double sh_d[256], sh_c[256];
+    __syncthreads();  // the arrays' previous use (a caller looping over blocks) is complete
     sh_d[threadIdx.x] = sd;
     sh_c[threadIdx.x] = sc;
     __syncthreads();
@@ -1306,10 +1318,36 @@ __device__ __forceinline__ void residual_block(const StepArgs& a, double* prev, 
         }
         __syncthreads();
     }
+    bd = sh_d[0];
+    bc = sh_c[0];
+}
+
+// SolutionChange from the sums over all blocks (gd: squared differences, gc:
+// squared values), and the loop's convergence stop (src/main.cpp:233-247).
+__device__ __forceinline__ void residual_result(const StepArgs& a, double gd, double gc, int loop, double threshold) {
+    const double nd = sqrt(gd), nc = sqrt(gc);
+    const double change = (nc > 1e-15) ? (nd / nc) : nd;
+    a.st->change = change;
+    if (loop) {
+        a.st->checked_step = a.st->step;
+        if (change < threshold) {
+            a.st->converged = 1;
+            a.st->done = 1;
+        }
+    }
+}
+
+// Diagnostics::SolutionChange + Fields::SavePrev (src/diagnostics.cpp:71-106,
+// src/fields.cpp:59-74) as block `bid` of `nblk`: the last block to finish
+// adds the block sums in block order.
+__device__ __forceinline__ void residual_block(const StepArgs& a, double* prev, double* partial, int loop,
+                                               int check_frequency, double threshold, int save, int bid, int nblk) {
+    double bd, bc;
+    residual_sums(a, prev, save, bid, nblk, bd, bc);
     __shared__ bool last;
     if (threadIdx.x == 0) {
-        partial[2 * bid] = sh_d[0];
-        partial[2 * bid + 1] = sh_c[0];
+        partial[2 * bid] = bd;
+        partial[2 * bid + 1] = bc;
         __threadfence();
         const unsigned t = atomicAdd(&a.st->ticket2, 1u);
         last = (t == (unsigned)nblk - 1);
@@ -1322,21 +1360,13 @@ __device__ __forceinline__ void residual_block(const StepArgs& a, double* prev, 
             gd += __hip_atomic_load(&partial[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             gc += __hip_atomic_load(&partial[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        const double nd = sqrt(gd), nc = sqrt(gc);
-        const double change = (nc > 1e-15) ? (nd / nc) : nd;
-        a.st->change = change;
-        if (loop) {
-            a.st->checked_step = a.st->step;
-            if (change < threshold) {
-                a.st->converged = 1;
-                a.st->done = 1;
-            }
-        }
+        residual_result(a, gd, gc, loop, threshold);
         a.st->ticket2 = 0;
     }
 }
 
-// A loop check runs only on a step that is a multiple of check_frequency
+// Whether the loop checks convergence after the step just finished: the step
+// count is a multiple of check_frequency, no rescan is pending,
 // and was not checked yet (src/main.cpp:233-247).
 __device__ __forceinline__ bool residual_due(const StepArgs& a, int check_frequency) {
     const int step = a.st->step;
@@ -1362,6 +1392,80 @@ __global__ void __launch_bounds__(256) lf_residual_batch(const StepArgs* __restr
     if (x.check_frequency <= 0 || !residual_due(a, x.check_frequency)) return;
     a.step1 = 0;
     residual_block(a, x.prev, x.partial, 1, x.check_frequency, x.threshold, 1, w.first, w.count);
+}
+
+// ------------------------------------------- persistent small-instance batch
+
+// The batched time loop of small instances in one launch: workgroup b runs up
+// to K steps of instance work[b].inst on its own, its four waves taking the
+// phases of each step in turn with workgroup barriers between them: the
+// central update over the instance's tiles (wave w takes tiles w, w + 4, ...),
+// the boundary ring (wave w takes the 64-cell groups w, w + 4, ... of the
+// compact ring layout, each wave with its own LDS work area behind the
+// instance's program image, staged once for the K steps), the step end
+// (thread 0), and, on the steps where the loop checks convergence,
+// SolutionChange + SavePrev over the single solver's partition of the
+// instance into 256-thread blocks, the block sums added in block order (the
+// same operations in the same order as lf_residual, so the same bits).  No
+// workgroup waits for another.  Step j of instance i reads
+// args[2 i + (icur[i] ^ (j & 1))]; an instance whose loop idles (done, rescan
+// pending, new state outside the split division's range) stops for the rest
+// of the launch, as in the graph of per-phase launches.
+template <bool kBuf, class Div>
+__global__ void __launch_bounds__(256) lf_batch_persist(const StepArgs* __restrict__ args,
+                                                       const int* __restrict__ icur,
+                                                       const BatchAux* __restrict__ aux,
+                                                       const BatchBlock* __restrict__ work, int K, int img_words,
+                                                       int area_slots) {
+    extern __shared__ uint64_t bc_lds[];
+    __shared__ int s_flag;
+    const BatchBlock w = work[blockIdx.x];
+    const int inst = __builtin_amdgcn_readfirstlane(w.inst);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int c0 = icur[inst];
+    const BatchAux x = aux[inst];
+    {
+        const StepArgs a0 = args[2 * inst + c0];
+        for (int i = (int)threadIdx.x; i < a0.bc_words; i += (int)blockDim.x) bc_lds[i] = a0.bc[i];
+    }
+    double* area = reinterpret_cast<double*>(bc_lds + img_words) + (size_t)wave * area_slots * BC_THREADS;
+    for (int j = 0; j < K; ++j) {
+        const StepArgs a = args[2 * inst + (c0 ^ (j & 1))];
+        if (threadIdx.x == 0) s_flag = loop_idle(a) ? 0 : 1;
+        __syncthreads();
+        if (!s_flag) break;
+        for (int t = wave; t < w.count; t += 4) stencil_tile<true, kBuf, Div>(a, 0.0, 1, w.H, w.nstrips, t);
+        __syncthreads();
+        {
+            const double dt = step_dt(a, 0.0, 1);
+            const int nring = ring_lanes_compact(a.L, a.Mn);
+            for (int g = wave; g * BC_THREADS < nring; g += 4) {
+                const RingLane rl = ring_lane(a, g * BC_THREADS + (int)(threadIdx.x & (BC_THREADS - 1)), false);
+                bc_ring_block(a, dt, 1, 1, 0, bc_lds, area, false, rl);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            finish_instance(a);
+            s_flag = (x.check_frequency > 0 && residual_due(a, x.check_frequency)) ? 1 : 0;
+        }
+        __syncthreads();
+        if (s_flag) {
+            // The state this step wrote is the u_in of the other orientation.
+            StepArgs an = args[2 * inst + (c0 ^ ((j + 1) & 1))];
+            an.step1 = 0;
+            const int nblk = residual_blocks(an.L, an.Mn);
+            double gd = 0.0, gc = 0.0;
+            for (int b = 0; b < nblk; ++b) {
+                double bd, bc;
+                residual_sums(an, x.prev, 1, b, nblk, bd, bc);
+                gd += bd;
+                gc += bc;
+            }
+            if (threadIdx.x == 0) residual_result(an, gd, gc, 1, x.threshold);
+        }
+        __syncthreads();
+    }
 }
 
 // Max wave speed of the state u_in over the interior (or of the IC physical
@@ -1633,6 +1737,31 @@ hipError_t launch_residual_batch(const StepArgs* args, const int* icur, const Ba
     if (nwork <= 0) return hipSuccess;
     hipLaunchKernelGGL(lf_residual_batch, dim3((unsigned)nwork), dim3(256), 0, s, args, icur, aux, work, flip);
     return hipGetLastError();
+}
+
+size_t persist_lds_bytes(int img_words, int area_slots) {
+    return ((size_t)img_words + (size_t)4 * area_slots * BC_THREADS) * 8;
+}
+
+hipError_t launch_batch_persist(const StepArgs* args, const int* icur, const BatchAux* aux, const BatchBlock* work,
+                                int nwork, int K, bool fast_div, bool buf, int img_words, int area_slots,
+                                hipStream_t s) {
+    if (nwork <= 0 || K <= 0) return hipSuccess;
+    const size_t lds = persist_lds_bytes(img_words, area_slots);
+    const dim3 g((unsigned)nwork), b(256);
+    // Dynamic LDS above 64 KiB has to be allowed per kernel.
+    auto launch = [&](auto kernel) -> hipError_t {
+        if (lds > 65536) {
+            const hipError_t e =
+                hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kernel, g, b, lds, s, args, icur, aux, work, K, img_words, area_slots);
+        return hipGetLastError();
+    };
+    if (buf) return fast_div ? launch(lf_batch_persist<true, DivConst>) : launch(lf_batch_persist<true, DivExact>);
+    return fast_div ? launch(lf_batch_persist<false, DivConst>) : launch(lf_batch_persist<false, DivExact>);
 }
 
 hipError_t launch_speed_fallback(const StepArgs& a, hipStream_t s) {

@@ -48,7 +48,7 @@ hipError_t launch_residual(const StepArgs& a, double* prev, double* partial, int
 // Workgroups of one instance's SolutionChange reduction: every launch of it
 // (single solver, batch, rescan path) partitions the cells the same way, so
 // an instance's change has the same bits however it is run.
-inline int residual_blocks(int L, int Mn) {
+__host__ __device__ inline int residual_blocks(int L, int Mn) {
     const long long b = ((long long)L * Mn + 255) / 256;
     return (int)(b < 1 ? 1 : (b < 1024 ? b : 1024));
 }
@@ -65,6 +65,14 @@ hipError_t launch_bc_batch(const StepArgs* args, const int* icur, const BatchBlo
 hipError_t launch_finish_batch(const StepArgs* args, const int* icur, int n, int flip, hipStream_t s);
 hipError_t launch_residual_batch(const StepArgs* args, const int* icur, const BatchAux* aux, const BatchBlock* work,
                                  int nwork, int flip, hipStream_t s);
+// The persistent batched loop of small instances: one 256-thread workgroup
+// per work entry {instance, 0, its tile count, tile rows, strips}, up to K
+// steps of that instance per launch.  LDS: the largest program image
+// (img_words) and four work areas of area_slots slots per lane.
+size_t persist_lds_bytes(int img_words, int area_slots);
+hipError_t launch_batch_persist(const StepArgs* args, const int* icur, const BatchAux* aux, const BatchBlock* work,
+                                int nwork, int K, bool fast_div, bool buf, int img_words, int area_slots,
+                                hipStream_t s);
 // Tiles of one instance for tile height H (strips of central columns x row chunks).
 inline int stencil_strips(int Mn) { return (Mn - 1) / kStripCols + 1; }
 inline int stencil_tiles(int L, int Mn, int H) { return stencil_strips(Mn) * ((L + H - 1) / H); }
