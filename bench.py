@@ -233,8 +233,10 @@ def sweep_measure(steps=1000, warm=200, n=64, seed=12345):
 
 def warmstart_measure(train, test, epochs, L=96, M=39):
     """Reduced live run of the learned warm start: train on `train` solver
-    steady states, then time plain cold solves (8 timed, scaled), cold and
-    warm batched solves of `test` held-out cases to residual 1e-8."""
+    steady states, then time plain cold solves (8 timed, scaled) and cold
+    batched solves of `test` held-out cases to residual 1e-8, and warm
+    batched solves to 1e-9 (the warm start's stop rule); errors of both
+    against the cases' cold solves continued to residual 1e-12."""
     import warmstart as WS
     import surrogate as SG
     import sweep
@@ -252,13 +254,16 @@ def warmstart_measure(train, test, epochs, L=96, M=39):
                       32, 2e-3, log_every=max(1, epochs // 4))
     train_s = time.perf_counter() - t1
     r, _, _ = WS.evaluate(net, norm, sweep.sample_params(test, 2025), L, M, plain_sample=8,
-                          budget=WS.step_budget(steps[conv]))
+                          budget=WS.step_budget(steps[conv]), warm_levels=[1e-9], ref_threshold=1e-12)
     keep = ["plain_cold_total_s", "cold_batch_s", "warm_total_s", "speedup_vs_plain_cold", "speedup_vs_cold_batch",
             "steps_ratio_cold_over_warm", "fallbacks", "all_converged_warm", "warm_finish_p50_s",
-            "warm_finish_p99_s", "max_field_err_warm_vs_cold"]
+            "warm_finish_p99_s", "cold_finish_p50_s", "cold_finish_p99_s", "warm_threshold",
+            "max_err_cold_vs_reference", "max_err_warm_vs_reference", "max_field_err_warm_vs_reference"]
     out = {k: r[k] for k in keep if k in r}
-    out.update({"workload": f"{test} held-out sweep cases at {L}x{M}, residual 1e-8; network trained on "
-                            f"{int(conv.sum())} steady states, {epochs} epochs",
+    out["reference_states_reached"] = r.get("reference", {}).get("reached")
+    out.update({"workload": f"{test} held-out sweep cases at {L}x{M}: cold to residual 1e-8, warm to 1e-9, "
+                            f"references to 1e-12; network trained on {int(conv.sum())} steady states, "
+                            f"{epochs} epochs",
                 "generate_s": round(gen_s, 2), "train_s": round(train_s, 2), "final_loss": losses[-1]})
     return out
 
