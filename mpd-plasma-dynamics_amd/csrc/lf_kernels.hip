@@ -963,7 +963,8 @@ __device__ __forceinline__ void axis_overrides(const StepArgs& a, const MpdSeg& 
 // face in L_LO, M_HI, M_LO, L_HI order that covers it); G: the cell's
 // boundary geometry table entries.
 __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, int f, int l, int m, double dt,
-                                         double* u, const RingGeo& G) {
+                                         double* u, const RingGeo& G, const double* pre_o, const double* pre_n,
+                                         int pre_nl, int pre_nm) {
     // Neighbour of the face (src/face_bc.cpp order: L_LO, M_HI, M_LO, L_HI;
     // only corners see an earlier face at their neighbour).
     int ln, mn;
@@ -976,13 +977,20 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
     } else if (f == MPD_L_HI && mn == a.Mn && face_covers(img(a), MPD_M_HI, ln)) {
         pf = MPD_M_HI; pl = ln; pm = a.Mn; ql = ln; qm = a.Mn - 1;
     }
-    // Every load of the chain first: own state (only AxisLF conditions read
-    // it), neighbour state, and the earlier face's neighbour.
+    // The own and positional-neighbour states were loaded with the lane
+    // (ring_lane); the face neighbour where it is not the positional one, and
+    // the earlier face's neighbour, here.
     const CellSrc so = s0_src(a, l, m), sn = s0_src(a, ln, mn);
     const CellSrc sq = s0_src(a, pf >= 0 ? ql : ln, pf >= 0 ? qm : mn);
     double vo[10], vn[10], vq[10];
-    src_load(so, vo);
-    src_load(sn, vn);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) vo[k] = pre_o[k];
+    if (ln == pre_nl && mn == pre_nm) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) vn[k] = pre_n[k];
+    } else {
+        src_load(sn, vn);
+    }
     if (pf >= 0) src_load(sq, vq);
     // The face neighbour is the row neighbour for an L face, the column
     // neighbour for an M face; an earlier face's chain at a corner's
@@ -1084,17 +1092,42 @@ __device__ __forceinline__ void stage_image(const uint64_t* src, uint64_t* prog,
 // instances' steps in a kernel of its own instead (lf_finish_batch).
 // A boundary lane's cell and its geometry table entries, read before the
 // program image is staged (the loads share the staging's round trip).
+// A boundary lane's cell, its geometry table entries, and the start states of
+// the cell and of its face neighbour by position (row 1: (2, m), row L:
+// (L-1, m), column 1: (l, 2), column Mn: (l, Mn-1)), loaded when the lane is
+// set up, before the program image is staged, so that the loads overlap the
+// staging.  The face neighbour differs from the positional one only at the
+// corners (1, Mn) and (L, Mn) when M_HI owns them; bc_final loads it there.
 struct RingLane {
     bool valid;
     int l, m;
+    int nl, nm;
     RingGeo G;
+    double vo[10], vn[10];
 };
 
 __device__ __forceinline__ RingLane ring_lane(const StepArgs& a, int idx, bool grouped) {
     RingLane r;
     r.l = r.m = 0;
     r.valid = grouped ? ring_cell(idx, a.L, a.Mn, r.l, r.m) : ring_cell_compact(idx, a.L, a.Mn, r.l, r.m);
-    if (r.valid) load_ring_geo(a, r.l, r.m, r.G);
+    if (r.l == 1) {
+        r.nl = 2;
+        r.nm = r.m;
+    } else if (r.l == a.L) {
+        r.nl = a.L - 1;
+        r.nm = r.m;
+    } else if (r.m == 1) {
+        r.nl = r.l;
+        r.nm = 2;
+    } else {
+        r.nl = r.l;
+        r.nm = a.Mn - 1;
+    }
+    if (r.valid) {
+        load_ring_geo(a, r.l, r.m, r.G);
+        src_load(s0_src(a, r.l, r.m), r.vo);
+        src_load(s0_src(a, r.nl, r.nm), r.vn);
+    }
     return r;
 }
 
@@ -1126,7 +1159,7 @@ __device__ __forceinline__ void bc_ring_block(const StepArgs& a_in, double dt, i
         }
         double u[8];
         if (f >= 0) {
-            bc_final(a, w, f, l, m, dt, u, G);
+            bc_final(a, w, f, l, m, dt, u, G, rl.vo, rl.vn, rl.nl, rl.nm);
         } else if (m >= 2 && m <= Mn - 1) {
             // Uncovered row cell keeps its central value.
             const double* rb = a.rowbuf + (int64_t)(l == 1 ? 0 : 8) * a.pitch;
