@@ -269,6 +269,18 @@ def stalled(done, converged, change, change_mid, stall_change):
     return bool(change > stall_change and not (change < change_mid))
 
 
+def fallback_rule(change, least, steps, grow, budget):
+    """The warm run's fallback tests at a chunk end for one running case:
+    returns (the case's new smallest SolutionChange, rising: the residual is
+    not finite or above grow x that smallest value, over: past the step
+    budget).  grow / budget None: that test is off."""
+    if np.isfinite(change) and change > 0.0:
+        least = min(least, change)
+    rising = grow is not None and (not np.isfinite(change) or change > grow * least)
+    over = budget is not None and steps > budget
+    return least, bool(rising), bool(over)
+
+
 def step_budget(train_steps, factor=1.5):
     """Most steps a warm-started case may take before it is restarted cold:
     factor times the slowest cold solve among the training states (a warm
@@ -346,12 +358,15 @@ def reference_states(cfgs, L, M, ref_threshold, max_steps, chunk=20000):
 
 def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_change=1e-6, plain_sample=None,
              chunk=1000, fallback=True, pred=None, budget=None, warm_levels=None, ref_threshold=None,
-             ref_max_steps=2000000, max_steps=2000000, staged=False):
+             ref_max_steps=2000000, max_steps=2000000, staged=False, grow=10.0):
     """Cold-started plain solves (each case alone, mpd_run) against the
     surrogate-warm-started batched sweep, both to the same threshold.  A
     warm case whose SolutionChange is still above stall_change after
     stall_steps steps is restarted cold (fallback), and so is one still
-    running after `budget` steps (step_budget; None: no budget).
+    running after `budget` steps (step_budget; None: no budget), and one
+    whose SolutionChange rises above `grow` times the smallest value it has
+    reached (a warm state that excites a growing mode: its residual falls,
+    then climbs for good; None: no such rule).
     plain_sample: time only that many plain solves and scale to all cases
     (stated in the result).
     warm_levels: tighter convergence levels for the warm-started cases (the
@@ -429,17 +444,25 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     restarted = set(stall) | set(rejected)
     late = []
 
+    least = np.full(n, np.inf)  # smallest SolutionChange each warm case has reached
+    rising = []
+
     def over_budget(res):
-        # Step budget: a warm case still running past it starts again cold.
+        # Step budget: a warm case still running past it starts again cold;
+        # so does one whose residual has risen grow-fold above its minimum.
         out = []
-        if fallback and budget is not None:
-            for i, r in enumerate(res):
-                if not r["done"] and i not in restarted and steps[i] > budget:
-                    b.instance(i).cold_restart(cfgs[i])
-                    b.instance(i).set_option("convergence_threshold", threshold)
-                    restarted.add(i)
-                    late.append(i)
-                    out.append(i)
+        if not fallback:
+            return out
+        for i, r in enumerate(res):
+            if r["done"] or i in restarted:
+                continue
+            least[i], up, over = fallback_rule(r["change"], least[i], steps[i], grow, budget)
+            if up or over:
+                b.instance(i).cold_restart(cfgs[i])
+                b.instance(i).set_option("convergence_threshold", threshold)
+                restarted.add(i)
+                (rising if up else late).append(i)
+                out.append(i)
         return out
 
     level_fields, level_steps, level_wall = [], [], []
@@ -461,7 +484,7 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
     warm_fields = level_fields[-1]
     b.close()
     progress(f"warm batch: inference {infer:.3f}s, relax {warm_wall:.2f}s, {len(rejected)} rejected, "
-             f"{len(stall)} stalled, {len(late)} over the step budget")
+             f"{len(stall)} stalled, {len(late)} over the step budget, {len(rising)} with a rising residual")
 
     errs = np.array([field_errors(warm_fields[i], cold_fields[i]) for i in range(n)])
     pred_errs = np.array([field_errors(pred[i], cold_fields[i]) for i in range(n)])
@@ -477,8 +500,8 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
         "warm_steps_p50": float(np.median(steps)), "warm_steps_max": int(steps.max()),
         "steps_saved_mean": float(np.mean(cold_steps - steps)),
         "steps_ratio_cold_over_warm": float(np.sum(cold_steps) / np.sum(steps)),
-        "fallbacks": len(rejected) + len(stall) + len(late), "rejected_predictions": len(rejected),
-        "stalled": len(stall), "over_budget": len(late), "step_budget": budget,
+        "fallbacks": len(rejected) + len(stall) + len(late) + len(rising), "rejected_predictions": len(rejected),
+        "stalled": len(stall), "over_budget": len(late), "rising": len(rising), "step_budget": budget,
         "all_converged_cold": bool(all(r["converged"] for r in cold_res)),
         "all_converged_warm": bool(all(r["converged"] for r in res)),
         "speedup_vs_plain_cold": plain_total / (infer + warm_wall),

@@ -136,3 +136,23 @@ def test_resample_fields_bilinear_exact():
                                    rtol=1e-13)
     np.testing.assert_allclose(g[1], -g[0], rtol=0, atol=0)
     assert np.array_equal(WS.resample_fields(f, L0, M0, L0, M0), f)
+
+
+def test_fallback_rule_rising_residual_and_budget():
+    """A warm case restarts cold when its residual climbs grow-fold above the
+    smallest value it reached (a growing mode), or is not finite, or when it
+    runs past the step budget; a falling residual keeps running."""
+    ws = WS
+    least = np.inf
+    for c in (3e-4, 5e-6, 2.5e-7, 1.2e-7):
+        least, up, over = ws.fallback_rule(c, least, 1000, 10.0, 500000)
+        assert not up and not over
+    assert least == 1.2e-7
+    least, up, _ = ws.fallback_rule(1.1e-6, least, 1000, 10.0, None)
+    assert not up and least == 1.2e-7  # 9.2x: still running
+    _, up, _ = ws.fallback_rule(1.3e-6, least, 1000, 10.0, None)
+    assert up
+    _, up, _ = ws.fallback_rule(float("nan"), least, 1000, 10.0, None)
+    assert up
+    _, up, over = ws.fallback_rule(1e-3, least, 600001, None, 600000)
+    assert not up and over
