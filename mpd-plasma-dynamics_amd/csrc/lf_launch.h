@@ -37,7 +37,8 @@ struct StencilCfg {
 // done flag from the device loop state instead of dt_explicit.
 hipError_t launch_stencil(const StepArgs& a, const StencilCfg& c, double dt_explicit, int loop, int speed,
                           hipStream_t s);
-// Boundary faces; with loop the last block finishes the step's loop state.
+// Boundary faces; with loop, a one-thread kernel after them finishes the
+// step's loop state (rescan test, dt bookkeeping).
 hipError_t launch_bc(const StepArgs& a, double dt_explicit, int loop, int speed, hipStream_t s);
 // Exact max-speed rescan + dt bookkeeping when the boundary kernel flagged
 // that the step's candidate max is not provably the max (loop only).
