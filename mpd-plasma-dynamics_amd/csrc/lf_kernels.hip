@@ -1033,17 +1033,37 @@ __device__ __forceinline__ void bc_final(const StepArgs& a, const WorkArea& w, i
     }
 }
 
+// The loop-state fields the end of a step reads, all loaded before any is
+// written (one round trip instead of a chain of dependent ones).
+struct LoopRead {
+    double t, dt_use, max_speed;
+    uint64_t bits;
+    int32_t step, max_steps, idle;
+    uint32_t range;
+};
+
+__device__ __forceinline__ LoopRead read_loop(const MpdLoopState* st) {
+    LoopRead r;
+    r.bits = __hip_atomic_load(&st->speed_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r.range = __hip_atomic_load(&st->range_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r.t = st->t;
+    r.dt_use = st->dt_use;
+    r.max_speed = st->max_speed;
+    r.step = st->step;
+    r.max_steps = st->max_steps;
+    r.idle = st->done | st->fallback | st->range_stop;
+    return r;
+}
+
 // dt bookkeeping of the time loop after one step: t += dt; dt =
 // ComputeDt(...) when adaptive (src/diagnostics.cpp:49-65); the next step's
 // dt truncated to T (src/main.cpp:220-223).
-__device__ void finalize_step(const StepArgs& a) {
+__device__ void finalize_from(const StepArgs& a, const LoopRead& r) {
     MpdLoopState* st = a.st;
-    const uint64_t bits = __hip_atomic_load(&st->speed_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double speed = __longlong_as_double((long long)bits);
-    st->max_speed = speed;
-    const double dt_used = st->dt_use;
-    st->t = st->t + dt_used;
-    st->step = st->step + 1;
+    const double speed = __longlong_as_double((long long)r.bits);
+    const double dt_used = r.dt_use;
+    const double t = r.t + dt_used;
+    const int step = r.step + 1;
     double dt = dt_used;
     if (a.adaptive) {
         const double dt_cfl = a.cfl_dx / (speed + 1.0e-10);
@@ -1051,16 +1071,21 @@ __device__ void finalize_step(const StepArgs& a) {
         const double v = (dt_grown < dt_cfl) ? dt_grown : dt_cfl;
         dt = (v < a.dt_min) ? a.dt_min : ((a.dt_max < v) ? a.dt_max : v);
     }
+    const bool cap = r.max_steps > 0 && step >= r.max_steps;
+    st->max_speed = speed;
+    st->t = t;
+    st->step = step;
     st->dt = dt;
-    st->dt_use = (st->t + dt > a.T) ? a.T - st->t : dt;
-    const bool cap = st->max_steps > 0 && st->step >= st->max_steps;
-    st->done = (!(st->t < a.T) || cap) ? 1 : 0;
+    st->dt_use = (t + dt > a.T) ? a.T - t : dt;
+    st->done = (!(t < a.T) || cap) ? 1 : 0;
     st->speed_bits = 0;
     st->ticket = 0;
     // A new state outside the split division's range: the loop idles after
     // this (complete) step until the host switches to IEEE division.
-    if (__hip_atomic_load(&st->range_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) st->range_stop = 1;
+    if (r.range != 0) st->range_stop = 1;
 }
+
+__device__ void finalize_step(const StepArgs& a) { finalize_from(a, read_loop(a.st)); }
 
 // Copies the program image (words 8-byte words) into `prog` (LDS).  Every
 // thread issues all its loads before its LDS stores: one global round trip
@@ -1228,22 +1253,25 @@ __global__ void __launch_bounds__(BC_THREADS) lf_bc_ring_batch(const StepArgs* _
 // atomics, so no per-workgroup ticket or device-scope fence is needed.
 // The end of one instance's step once all its cells are written: the
 // exact-rescan test, then the dt bookkeeping (finalize_step).
-__device__ __forceinline__ void finish_instance(const StepArgs& a) {
-    const double T = speed_T(a, 1);
-    const uint64_t bits = __hip_atomic_load(&a.st->speed_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double mc = __longlong_as_double((long long)bits);
+// r: the loop state read at the step's end (read_loop).
+__device__ __forceinline__ void finish_instance(const StepArgs& a, const LoopRead& r) {
+    // speed_T with the loop state already read.
+    const double Tr = r.max_speed * (1.0 - SPEED_DROP);
+    const double T = (Tr > 1.0e-30) ? Tr : 0.0;
+    const double mc = __longlong_as_double((long long)r.bits);
     if (T > 0.0 && !(mc >= T)) {
         a.st->fallback = 1;
         a.st->ticket = 0;
     } else {
-        finalize_step(a);
+        finalize_from(a, r);
     }
 }
 
 // The step's end of a single grid's loop step, after lf_bc_ring.
 __global__ void lf_finish_one(StepArgs a) {
-    if (loop_idle(a)) return;
-    finish_instance(a);
+    const LoopRead r = read_loop(a.st);
+    if (r.idle) return;
+    finish_instance(a, r);
 }
 
 __global__ void __launch_bounds__(64) lf_finish_batch(const StepArgs* __restrict__ args, const int* __restrict__ icur,
@@ -1251,8 +1279,9 @@ __global__ void __launch_bounds__(64) lf_finish_batch(const StepArgs* __restrict
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     const StepArgs a = args[2 * i + (icur[i] ^ flip)];
-    if (loop_idle(a)) return;
-    finish_instance(a);
+    const LoopRead r = read_loop(a.st);
+    if (r.idle) return;
+    finish_instance(a, r);
 }
 
 // Exact max wave speed over the whole new state, run when the candidate max
@@ -1461,7 +1490,7 @@ __global__ void __launch_bounds__(256) lf_batch_persist(const StepArgs* __restri
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            finish_instance(a);
+            finish_instance(a, read_loop(a.st));
             s_flag = (x.check_frequency > 0 && residual_due(a, x.check_frequency)) ? 1 : 0;
         }
         __syncthreads();
