@@ -15,7 +15,10 @@ boundary faces, CFL dt update), with the state resident in HBM.
 
 Also on the line: `batched_sweep` (64 uneven instances at the sweep sizes,
 batched against one at a time), `steady_state` (a reduced live run of the
-learned warm start; --no-warmstart skips it), the measured copy ceiling of
+learned warm start at 96x39: generation, training and evaluation) and
+`steady_state_400x160` (time to steady state on the warm start's own grid,
+400x160, with the committed network trained at that grid; --no-warmstart
+skips both, --c3-test 0 the second), the measured copy ceiling of
 the HBM, and `cpu_baseline`: the reference's own compiled step
 (oracle/_ref/ref_harness, built from the reference sources in the build
 container; it travels with the tree) on a bounded sample of the same
@@ -268,6 +271,37 @@ def warmstart_measure(train, test, epochs, L=96, M=39):
     return out
 
 
+WARMSTART_NET = os.path.join("mpd-plasma-dynamics_amd", "data", "warmstart_400x160.npz")
+
+
+def warmstart_c3_measure(test, plain, ref_steps, L=400, M=160):
+    """Time to steady state on the warm start's own grid (400x160) with the
+    committed network (trained by the project's kernels on 2000 solver steady
+    states at this grid; profiles/warmstart_400x160_training.json): `test`
+    held-out cases (the first of the 64 of the recorded evaluation), plain
+    cold solves one at a time (`plain` timed, scaled), cold batched to 1e-8,
+    warm batched to 1e-9, and both against the cases' cold solves continued
+    to 1e-12 (at most `ref_steps` more steps; the cases that reach it)."""
+    import warmstart as WS
+    import sweep
+    path = os.path.join(REPO, WARMSTART_NET)
+    net, norm, train_steps = WS.load_network(path, L, M)
+    r, _, _ = WS.evaluate(net, norm, sweep.sample_params(test, 2025), L, M, plain_sample=plain,
+                          budget=WS.step_budget(train_steps), warm_levels=[1e-9], ref_threshold=1e-12,
+                          ref_max_steps=ref_steps, max_steps=600000)
+    keep = ["plain_cold_total_s", "cold_batch_s", "warm_total_s", "speedup_vs_plain_cold", "speedup_vs_cold_batch",
+            "steps_ratio_cold_over_warm", "fallbacks", "all_converged_warm", "warm_finish_p50_s",
+            "warm_finish_p99_s", "cold_finish_p50_s", "cold_finish_p99_s", "warm_threshold",
+            "max_err_cold_vs_reference", "max_err_warm_vs_reference", "max_field_err_warm_vs_reference"]
+    out = {k: r[k] for k in keep if k in r}
+    out["reference_states_reached"] = r.get("reference", {}).get("reached")
+    out["workload"] = (f"{test} held-out sweep cases at {L}x{M} (config 3 grid): plain cold solves one at a time "
+                       f"({plain} timed, scaled), cold batched to residual 1e-8, warm batched to 1e-9, references "
+                       f"to 1e-12 (at most {ref_steps} more steps); every case capped at 600000 steps; "
+                       f"network {WARMSTART_NET}")
+    return out
+
+
 def traffic_record():
     """HBM bytes per stencil launch from the committed counter profile (two
     rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy
@@ -342,6 +376,12 @@ def mode_step(args):
     if not args.no_warmstart:
         print("bench: learned warm start (reduced live run)", file=sys.stderr, flush=True)
         line["steady_state"] = warmstart_measure(args.warmstart_train, args.warmstart_test, args.warmstart_epochs)
+        if args.c3_test > 0:
+            print("bench: learned warm start at 400x160 (committed network)", file=sys.stderr, flush=True)
+            try:
+                line["steady_state_400x160"] = warmstart_c3_measure(args.c3_test, args.c3_plain, args.c3_ref_steps)
+            except Exception as e:  # the headline line is printed whatever happens here
+                line["steady_state_400x160"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(L, M, args.cpu_steps, args.cpu_steps_1t, args.ref_steps,
@@ -418,6 +458,10 @@ def main():
     ap.add_argument("--warmstart-test", type=int, default=64)
     ap.add_argument("--warmstart-epochs", type=int, default=60)
     ap.add_argument("--no-warmstart", action="store_true")
+    ap.add_argument("--c3-test", type=int, default=32,
+                    help="held-out cases of the 400x160 warm-start measurement (0: skip it)")
+    ap.add_argument("--c3-plain", type=int, default=4, help="plain cold solves timed there")
+    ap.add_argument("--c3-ref-steps", type=int, default=200000, help="reference continuation cap there")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--probe", choices=["small", "uneven"], default="small")
     ap.add_argument("--persistent", action="store_true",

@@ -558,13 +558,31 @@ def evaluate(net, norm, params, L, M, threshold=1e-8, stall_steps=2000, stall_ch
 
 def load_network(path, L, M, seed=2024):
     """A network and normalizer saved by experiment(out=...) (its
-    warmstart_net.npz), with the training cases' cold step counts."""
+    warmstart_net.npz, FP32 weights w<k>) or by save_network_bf16 (bf16 weight
+    bit patterns wb<k>), with the training cases' cold step counts."""
     z = np.load(path)
     net = SG.UNet(L, M + 1, seed=seed)
     nl = len(net.layers)
-    net.load_params([z[f"w{k}"] for k in range(nl)], [z[f"b{k}"] for k in range(nl)])
+    if "wb0" in z:
+        ws = [(z[f"wb{k}"].astype(np.uint32) << 16).view(np.float32) for k in range(nl)]
+    else:
+        ws = [z[f"w{k}"] for k in range(nl)]
+    net.load_params(ws, [z[f"b{k}"] for k in range(nl)])
     norm = Normalizer(None, mean=z["mean"], std=z["std"])
     return net, norm, z["train_steps"] if "train_steps" in z else np.array([])
+
+
+def save_network_bf16(src, dst):
+    """Rewrites a warmstart_net.npz with its weights as the bf16 bit patterns
+    the kernels compute with (round to nearest even of the FP32 masters, the
+    rounding of the weights' bf16 copies), biases, normalizer and training
+    step counts as they are: half the size, the same predictions."""
+    z = np.load(src)
+    nl = sum(1 for k in z.files if k.startswith("w") and k[1:].isdigit())
+    out = {f"wb{k}": SG.bf16_bits(z[f"w{k}"]) for k in range(nl)}
+    out.update({f"b{k}": z[f"b{k}"].astype(np.float32) for k in range(nl)})
+    out.update(mean=z["mean"], std=z["std"], train_steps=z["train_steps"])
+    np.savez_compressed(dst, **out)
 
 
 def experiment(L, M, train_n=2000, test_n=64, epochs=150, batch=32, lr=2e-3, clip=None, plain_sample=16,

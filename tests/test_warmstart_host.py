@@ -156,3 +156,31 @@ def test_fallback_rule_rising_residual_and_budget():
     assert up
     _, up, over = ws.fallback_rule(1e-3, least, 600001, None, 600000)
     assert not up and over
+
+
+def test_network_file_bf16(tmp_path):
+    """save_network_bf16 keeps the weights as the bf16 bit patterns of the
+    FP32 masters (round to nearest even, the kernels' rounding of the
+    weights' bf16 copies), which decode to FP32 values whose bf16 rounding is
+    themselves: the network computes with the same bf16 weights."""
+    rng = np.random.default_rng(5)
+    w = [rng.standard_normal((4, 9 * 3)).astype(np.float32) for _ in range(2)]
+    # Halfway cases of the bf16 rounding, both parities of the kept bit.
+    w[0][0, :4] = np.array([0x3F808000, 0x3F818000, 0xBF808001, 0x7F7FFFFF], dtype=np.uint32).view(np.float32)
+    b = [rng.standard_normal(4).astype(np.float32) for _ in range(2)]
+    src, dst = tmp_path / "net.npz", tmp_path / "net_bf16.npz"
+    np.savez(src, w0=w[0], w1=w[1], b0=b[0], b1=b[1], mean=np.arange(8.0), std=np.ones(8),
+             train_steps=np.array([10, 20]))
+    WS.save_network_bf16(src, dst)
+    z = np.load(dst)
+    assert sorted(z.files) == ["b0", "b1", "mean", "std", "train_steps", "wb0", "wb1"]
+    assert z["wb0"][0, :4].tolist() == [0x3F80, 0x3F82, 0xBF81, 0x7F80]
+    for k in range(2):
+        assert np.array_equal(z[f"wb{k}"], SG.bf16_bits(w[k]))
+        dec = (z[f"wb{k}"].astype(np.uint32) << 16).view(np.float32)
+        assert np.array_equal(SG.bf16_bits(dec), z[f"wb{k}"])
+        fin = np.isfinite(dec)  # the largest float rounds to +inf, as in the kernels
+        assert fin.sum() == w[k].size - (k == 0)
+        assert np.all(np.abs(dec - w[k])[fin] <= np.abs(w[k])[fin] * 2.0 ** -8)
+        assert np.array_equal(z[f"b{k}"], b[k])
+    assert np.array_equal(z["train_steps"], [10, 20])
