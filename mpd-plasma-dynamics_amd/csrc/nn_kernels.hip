@@ -11,9 +11,13 @@
 //   weight grad  m = output channel, n = (tap, input channel), k = pixel
 //                (split over pixel ranges; partial sums reduced in a fixed
 //                order, so the result is deterministic)
-// A 256-thread workgroup computes a 64 x 64 tile of C: operand tiles of
-// 64 x 32 staged through LDS, each wave a 32 x 32 sub-tile as 2 x 2
-// v_mfma_f32_16x16x16_bf16 accumulators.
+// conv_gemm (any channel count): a 256-thread workgroup computes a 64 x 64
+// tile of C from element-staged 64 x 32 operand tiles, each wave a 32 x 32
+// sub-tile as 2 x 2 v_mfma_f32_16x16x16_bf16 accumulators.  conv_gemm2
+// (channel counts multiples of 8): tiles up to 128 x 128 on
+// v_mfma_f32_16x16x32_bf16 from 16-byte runs.  wgrad_lds (the weight
+// gradient, channel counts multiples of 16): row blocks staged in LDS and
+// read with gfx950's LDS transpose reads.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -516,6 +520,167 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
             }
 }
 
+// The weight gradient from LDS-resident row blocks, for channel counts that
+// are multiples of 16: dw[co][tap][ci] = sum over pixels p of dz[p][co]
+// x[p + shift(tap)][ci].  The pixel sum is the GEMM's k, against the NHWC
+// layout of both operands.  Workgroup (co tile, ci tile, slice) takes rows
+// y0 .. y1 - 1 of one image in chunks of RC rows: the chunk's dz rows (one
+// LDS row per pixel, CO_T channels) and its x rows with a halo row and
+// column on every side (zero outside the image, CI_T = 16 channels) are
+// staged with 16-byte stores, and gfx950's LDS transpose read
+// (ds_read_b64_tr_b16) hands the matrix cores each operand as 8 consecutive
+// pixels of one channel: in a 16-lane group, lane s reads pixel k0 + s / 4
+// at channels 4 (s % 4) .. + 3, and output lane 4 a + b receives channel b
+// of the reads of lanes a, a + 4, a + 8, a + 12 -- pixels k0 .. k0 + 3 of
+// channel 4 a + b (lane layout measured on the GPU).  Two such reads make the
+// 8 k values of v_mfma_f32_16x16x32_bf16's operand.  A tap's shift is an
+// address offset into the staged x rows, so the 9 taps read one tile.  Wave
+// w accumulates every co fragment against taps w, w + 4, w + 8; the slice's
+// sums go to out[slice] for the fixed-order column sum (deterministic).
+// Pixels of a chunk past its last row (the k padding to a multiple of 32)
+// have dz = 0 and read the chunk's last pixel's x.
+struct WgArgs {
+    const uint16_t* x;   // [B][H][W][Cin]
+    const uint16_t* dz;  // [B][H][W][Cout]
+    float* out;          // partial sums [S][Cout][9][Cin]
+    int B, H, W, Cin, Cout;
+    int RC;   // rows per chunk
+    int rps;  // rows per slice (a multiple of RC)
+    int spi;  // slices per image
+};
+
+constexpr int WG_CI = 16, WG_XP = WG_CI + 8;  // ci per workgroup; x tile row pitch (bf16, 48 bytes)
+constexpr int WG_RUNS = 8;                     // staging runs per thread per tile, at most
+__host__ __device__ constexpr int wg_dp(int co_t) { return co_t + 8; }  // dz tile row pitch
+
+__device__ __forceinline__ bf16x4 tr_read(const uint16_t* p) {
+    typedef __attribute__((address_space(3))) bf16x4 lds_v4;
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4*)p);
+}
+
+__device__ __forceinline__ bf16x8v join8(bf16x4 lo, bf16x4 hi) {
+    return __builtin_bit_cast(bf16x8v, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
+template <int CO_T>
+__global__ void __launch_bounds__(256) wgrad_lds(WgArgs a) {
+    constexpr int DP = wg_dp(CO_T), FC = CO_T / 16, NW = 3;  // taps per wave: w, w + 4, w + 8 (< 9)
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+    const int W = a.W, W2 = W + 2, RC = a.RC;
+    const int KQ = (RC * W + 31) / 32 * 32;
+    uint16_t* xs = lds;
+    uint16_t* ds = lds + (RC + 2) * W2 * WG_XP;
+    const int co0 = (int)blockIdx.x * CO_T, ci0 = (int)blockIdx.y * WG_CI;
+    const int sl = (int)blockIdx.z, b = sl / a.spi;
+    const int y0 = (sl - b * a.spi) * a.rps, y1 = min(a.H, y0 + a.rps);
+    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int s = lane & 15, g = lane >> 4;
+    const int qlane = 8 * g + (s >> 2);  // this lane's pixel in a k-step (half 0; +4 for half 1)
+    const int cl = 4 * (s & 3);          // its channel offset within a fragment
+    f32x4 acc[FC][NW];
+#pragma unroll
+    for (int i = 0; i < FC; ++i)
+#pragma unroll
+        for (int j = 0; j < NW; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int64_t img = (int64_t)b * a.H;
+    // Staging runs (16 bytes) of one chunk: x rows yc - 1 .. yc + nr,
+    // columns -1 .. W, two 8-channel runs each; dz pixels 0 .. KQ - 1 (zero
+    // past the chunk's last row).  Thread t moves runs t + 256 i (the plan
+    // keeps both tiles within 256 WG_RUNS runs).  The next chunk's runs are
+    // loaded into registers while the current chunk is computed.
+    constexpr int RD = CO_T / 8;
+    uint4 rx[WG_RUNS], rd[WG_RUNS];
+    auto fetch = [&](int yc) {
+        const int nr = min(RC, y1 - yc), np = nr * W, nx = (nr + 2) * W2 * 2;
+#pragma unroll
+        for (int i = 0; i < WG_RUNS; ++i) {
+            const int r = t + 256 * i;
+            const int pos = r >> 1, j = r & 1;
+            const int row = pos / W2, col = pos - row * W2;
+            const int y = yc - 1 + row, x = col - 1;
+            rx[i] = (r < nx && y >= 0 && y < a.H && x >= 0 && x < W)
+                        ? *reinterpret_cast<const uint4*>(a.x + ((img + y) * W + x) * a.Cin + ci0 + 8 * j)
+                        : zero_run();
+            const int q = r / RD, jd = r - q * RD;
+            const int yy = q / W, xx = q - yy * W;
+            rd[i] = (q < np) ? *reinterpret_cast<const uint4*>(a.dz + ((img + yc + yy) * W + xx) * a.Cout + co0 +
+                                                               8 * jd)
+                             : zero_run();
+        }
+    };
+    auto put = [&](int yc) {
+        const int nr = min(RC, y1 - yc), nx = (nr + 2) * W2 * 2;
+#pragma unroll
+        for (int i = 0; i < WG_RUNS; ++i) {
+            const int r = t + 256 * i;
+            if (r < nx) *reinterpret_cast<uint4*>(xs + (r >> 1) * WG_XP + 8 * (r & 1)) = rx[i];
+            const int q = r / RD;
+            if (q < KQ) *reinterpret_cast<uint4*>(ds + q * DP + 8 * (r - q * RD)) = rd[i];
+        }
+    };
+    fetch(y0);
+    for (int yc = y0; yc < y1; yc += RC) {
+        const int nr = min(RC, y1 - yc), np = nr * W;
+        put(yc);
+        __syncthreads();
+        if (yc + RC < y1) fetch(yc + RC);
+        // This lane's pixels in the k-step (two halves) as (row, column) of
+        // the chunk, advanced by 32 pixels per step.
+        int py[2], px[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = min(qlane + 4 * h, np - 1);
+            py[h] = q / W;
+            px[h] = q - py[h] * W;
+        }
+        for (int kb = 0; kb < KQ; kb += 32) {
+            bf16x8v fa[FC], fb[NW];
+#pragma unroll
+            for (int i = 0; i < FC; ++i) {
+                const uint16_t* p0 = ds + (kb + qlane) * DP + 16 * i + cl;
+                fa[i] = join8(tr_read(p0), tr_read(p0 + 4 * DP));
+            }
+#pragma unroll
+            for (int j = 0; j < NW; ++j) {
+                const int tap = wave + 4 * j;
+                if (tap < 9) {
+                    const int ky = tap / 3, kx = tap - 3 * ky;
+                    const uint16_t* p0 = xs + ((py[0] + ky) * W2 + px[0] + kx) * WG_XP + cl;
+                    const uint16_t* p1 = xs + ((py[1] + ky) * W2 + px[1] + kx) * WG_XP + cl;
+                    fb[j] = join8(tr_read(p0), tr_read(p1));
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < FC; ++i)
+#pragma unroll
+                for (int j = 0; j < NW; ++j)
+                    if (wave + 4 * j < 9)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            // Next k-step: 32 pixels on, clamped to the chunk's last pixel.
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = min(kb + 32 + qlane + 4 * h, np - 1);
+                py[h] = q / W;
+                px[h] = q - py[h] * W;
+            }
+        }
+        __syncthreads();
+    }
+    float* out = a.out + (int64_t)sl * a.Cout * 9 * a.Cin;
+#pragma unroll
+    for (int i = 0; i < FC; ++i)
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+            const int tap = wave + 4 * j;
+            if (tap >= 9) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int co = co0 + 16 * i + 4 * g + e, ci = ci0 + s;
+                out[((int64_t)co * 9 + tap) * a.Cin + ci] = acc[i][j][e];
+            }
+        }
+}
+
 // out[i] = sum over rows s < S of part[s][i], i < n: a block of 16 waves
 // per 64 outputs, wave w summing rows w, w + 16, ..., then the 16 partial
 // sums added in a fixed tree order (deterministic, whatever the hardware
@@ -790,6 +955,42 @@ int wgrad_splits(int P, int M, int N, bool vec) {
 }
 int wgrad_chunk(int P, int S) { return ((P + S - 1) / S + BK - 1) / BK * BK; }
 
+// Launch plan of wgrad_lds: channel counts multiples of 16, rows per chunk
+// for about 512 pixels with the LDS tiles within 60 KiB (two workgroups per
+// CU), slices of whole chunks so that about 1024 workgroups run; false when
+// a single row's tiles do not fit (very wide images: the GEMM path then).
+struct WgPlan {
+    int co_t, RC, rps, spi, S;
+    size_t lds;
+};
+size_t wgrad_lds_bytes(int W, int RC, int co_t) {
+    return ((size_t)(RC + 2) * (W + 2) * WG_XP + (size_t)((RC * W + 31) / 32 * 32) * wg_dp(co_t)) * 2;
+}
+bool wgrad_lds_plan(int B, int H, int W, int Cin, int Cout, WgPlan& p) {
+    if (Cin % WG_CI != 0 || Cout % 16 != 0) return false;
+    p.co_t = (Cout % 32 == 0) ? 32 : 16;
+    constexpr size_t kMaxLds = 60 * 1024;
+    int RC = std::max(1, std::min(H, 512 / W));
+    // Both staged tiles within 256 WG_RUNS 16-byte runs (the kernel's
+    // per-thread staging registers).
+    auto fits = [&](int rc) {
+        return wgrad_lds_bytes(W, rc, p.co_t) <= kMaxLds && (rc + 2) * (W + 2) * 2 <= 256 * WG_RUNS &&
+               (rc * W + 31) / 32 * 32 * (p.co_t / 8) <= 256 * WG_RUNS;
+    };
+    while (RC > 1 && !fits(RC)) --RC;
+    if (!fits(RC)) return false;
+    p.RC = RC;
+    p.lds = wgrad_lds_bytes(W, RC, p.co_t);
+    const int tiles = (Cout / p.co_t) * (Cin / WG_CI);
+    const int chunks = (H + RC - 1) / RC;
+    const int want = std::max(1, 1024 / (tiles * B));
+    const int cps = (chunks + std::min(chunks, want) - 1) / std::min(chunks, want);
+    p.rps = cps * RC;
+    p.spi = (H + p.rps - 1) / p.rps;
+    p.S = B * p.spi;
+    return true;
+}
+
 // Launch of conv_gemm2 for C = a.M x a.N with the tile pick_tile chooses.
 template <int MODE>
 void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
@@ -884,6 +1085,8 @@ int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, in
 
 long long mpd_nn_wgrad_work_floats(int B, int H, int W, int Cin, int Cout) {
     if (!conv_dims_ok(B, H, W, Cin, Cout)) return -1;
+    WgPlan pl;
+    if (wgrad_lds_plan(B, H, W, Cin, Cout, pl)) return (long long)pl.S * Cout * 9 * Cin;
     const int P = B * H * W;
     return (long long)wgrad_splits(P, Cout, 9 * Cin, vec_ok(Cin, Cout)) * Cout * 9 * Cin;
 }
@@ -896,6 +1099,18 @@ int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, 
     a.dz = dz;
     a.out = work;
     a.B = B, a.H = H, a.W = W, a.Cin = Cin, a.Cout = Cout;
+    hipStream_t s = (hipStream_t)stream;
+    WgPlan pl;
+    if (wgrad_lds_plan(B, H, W, Cin, Cout, pl)) {
+        WgArgs w{x, dz, work, B, H, W, Cin, Cout, pl.RC, pl.rps, pl.spi};
+        const dim3 g((unsigned)(Cout / pl.co_t), (unsigned)(Cin / WG_CI), (unsigned)pl.S);
+        if (pl.co_t == 32) hipLaunchKernelGGL(wgrad_lds<32>, g, dim3(256), pl.lds, s, w);
+        else hipLaunchKernelGGL(wgrad_lds<16>, g, dim3(256), pl.lds, s, w);
+        const int64_t n = (int64_t)Cout * 9 * Cin;
+        hipLaunchKernelGGL(column_sum, dim3((unsigned)((n + 63) / 64)), dim3(1024), 0, s, (const float*)work, dw, n,
+                           pl.S);
+        return finish(hipSuccess);
+    }
     a.M = Cout;
     a.N = 9 * Cin;
     a.K = B * H * W;
@@ -903,7 +1118,6 @@ int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, 
     const int S = wgrad_splits(a.K, a.M, a.N, vec);
     a.kchunk = wgrad_chunk(a.K, S);
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN), (unsigned)S);
-    hipStream_t s = (hipStream_t)stream;
     if (vec) launch_gemm2<MODE_WGRAD>(a, S, s);
     else hipLaunchKernelGGL((conv_gemm<MODE_WGRAD, false>), g, dim3(256), 0, s, a);
     const int64_t n = (int64_t)a.M * a.N;
