@@ -139,8 +139,9 @@ def probe_configs(kind, seed=7):
 def converge_uneven(mpd_amd, sizes, n=64, seed=2, threshold=1e-8, plain_sample=16, chunk=2000):
     """n seeded cases with nz drawn uniformly from `sizes` (nr = 0.4 nz, the
     parameter box of sample_params), every case run to SolutionChange <
-    threshold: once batched, once one case at a time (plain_sample cases
-    timed, scaled to n by the mean of each size).  Returns a dict."""
+    threshold or to 3 M steps: once batched, once one case at a time
+    (plain_sample cases timed, scaled to n by the mean of each size).
+    Returns a dict (with how many cases converged each way)."""
     import json  # noqa: F401  (the caller prints)
     rng = np.random.default_rng(seed)
     nz = rng.choice(np.asarray(sizes), size=n)
@@ -171,6 +172,7 @@ def converge_uneven(mpd_amd, sizes, n=64, seed=2, threshold=1e-8, plain_sample=1
         conv = all(r["converged"] for r in res)
     # One case at a time: the same configs, sampled evenly over the sizes.
     per_size = {}
+    plain_conv = 0
     order = np.argsort(nz, kind="stable")
     pick = order[np.linspace(0, n - 1, min(n, plain_sample)).astype(int)]
     for i in pick:
@@ -178,14 +180,16 @@ def converge_uneven(mpd_amd, sizes, n=64, seed=2, threshold=1e-8, plain_sample=1
             t1 = time.perf_counter()
             r = sol.run()
             per_size.setdefault(int(nz[i]), []).append(time.perf_counter() - t1)
-            assert r["converged"]
-        print(f"[sweep] one case at a time: nz {nz[i]}, {r['steps']} steps, {per_size[int(nz[i])][-1]:.2f} s",
-              flush=True)
+            plain_conv += int(r["converged"])
+        print(f"[sweep] one case at a time: nz {nz[i]}, {r['steps']} steps, {per_size[int(nz[i])][-1]:.2f} s, "
+              f"converged {r['converged']}", flush=True)
     mean = {z: float(np.mean(v)) for z, v in per_size.items()}
     plain_s = float(sum(mean[int(z)] for z in nz))
     return {"cases": n, "sizes": [int(z) for z in sizes], "counts": [int(np.sum(nz == z)) for z in sizes],
             "cells": int(cells), "threshold": threshold, "all_converged": bool(conv),
+            "converged_batched": int(sum(r["converged"] for r in res)), "step_cap": 3000000,
             "batched_s": batch_s, "one_at_a_time_s": plain_s, "one_at_a_time_timed": int(len(pick)),
+            "one_at_a_time_converged": plain_conv,
             "one_at_a_time_mean_s_by_size": mean, "speedup": plain_s / batch_s,
             "steps_p50": float(np.median(steps)), "steps_max": int(steps.max()),
             "finish_p50_s": float(np.nanmedian(finish)), "finish_p99_s": float(np.nanpercentile(finish, 99))}
