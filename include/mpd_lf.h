@@ -156,7 +156,8 @@ int mpd_write_vtk_fields(const char* yaml_text, const double* const* phys, const
 /* Kernel selection: key "kernel" (0 = simple, 1 = tiled), "fast_div" (0/1),
  * "tile_rows", "tile_order" (0: strip-fastest launch order, 1: whole row bands
  * per XCD, 2: four vertically adjacent tiles per workgroup marching in
- * alternate directions, chunk groups per XCD), "buffers" (1: raw buffer
+ * alternate directions, chunk groups per XCD, 3: a group of strips per XCD
+ * taken chunk by chunk), "buffers" (1: raw buffer
  * addressing for states below 2^31 bytes, the default; 0: the plain-pointer
  * path larger states take), "steps_per_graph", "convergence_threshold" (a new
  * level for the loop's convergence stop, for a config with a convergence

@@ -518,6 +518,21 @@ __global__ void __launch_bounds__(256, 2) lf_stencil_tiled(StepArgs a, double dt
     if (order == 0) {
         tile = (int)blockIdx.x * 4 + wave;
         if (tile >= ntiles) return;
+    } else if (order == 3) {
+        // Order 3: XCD x (workgroup w goes to XCD w mod 8) owns strips
+        // s_lo(x) .. s_lo(x + 1) - 1, s_lo(x) = x * nstrips / 8, and takes
+        // their tiles chunk by chunk: the two tiles a halo row is shared by
+        // (the same strip, adjacent chunks) run on one XCD a few tiles
+        // apart, so the second read of the row can hit that XCD's L2, while
+        // the XCDs still advance together over a narrow band of rows.
+        const int w = (int)blockIdx.x;
+        const int x = w & 7, j = w >> 3;
+        const int s0 = x * nstrips / 8, cnt = (x + 1) * nstrips / 8 - s0;
+        if (cnt <= 0) return;
+        const int i = 4 * j + wave;
+        const int chunk = i / cnt;
+        if (chunk >= ntiles / nstrips) return;
+        tile = chunk * nstrips + s0 + (i - chunk * cnt);
     } else if (order == 2) {
         const int nchunks = ntiles / nstrips;
         const int w = (int)blockIdx.x;
@@ -1706,8 +1721,10 @@ hipError_t launch_tiled(const StepArgs& a, int H, int order, bool buffers, doubl
     const int nchunks = (a.L + H - 1) / H;
     const int ntiles = nstrips * nchunks;
     const int ngroups = (nchunks + 3) / 4;  // order 2: chunk groups of four
+    const int max_cnt = (nstrips + 7) / 8;  // order 3: strips of the XCD with the most
     const unsigned blocks = order == 0   ? (unsigned)((ntiles + 3) / 4)
                             : order == 1 ? (unsigned)(8 * ((nchunks + 7) / 8) * ((nstrips + 3) / 4))
+                            : order == 3 ? (unsigned)(8 * ((max_cnt * nchunks + 3) / 4))
                                          : (unsigned)(8 * ((ngroups + 7) / 8) * nstrips);
     // Buffer addressing needs every row/field offset of a state below 2^31 bytes.
     const bool buf = buffers && (double)a.fstride * 8.0 * 8.0 < 2147483648.0;

@@ -27,7 +27,7 @@ struct StencilCfg {
     int kind = STENCIL_TILED;
     bool fast_div = true;
     int tile_rows = 0;  // rows per wave of the tiled kernel (0: chosen from the grid)
-    int order = 0;      // tile order: 0 strip-fastest, 1 row bands per XCD, 2 stacked tiles (see lf_stencil_tiled)
+    int order = 0;      // tile order: 0 strip-fastest, 1 row bands per XCD, 2 stacked tiles, 3 strip groups per XCD (see lf_stencil_tiled)
     // Raw buffer addressing when every state is below 2^31 bytes (false:
     // always the plain-pointer path, which larger states take).
     bool buffers = true;

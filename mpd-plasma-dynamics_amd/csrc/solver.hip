@@ -248,7 +248,7 @@ int mpd_set_option(mpd_solver* s, const char* key, double value) {
         else if (k == "tile_rows") s->scfg.tile_rows = (int)value;
         else if (k == "tile_order") {
             const int o = (int)value;
-            if (o < 0 || o > 2) throw ArgError("tile_order must be 0, 1 or 2");
+            if (o < 0 || o > 3) throw ArgError("tile_order must be 0, 1, 2 or 3");
             s->scfg.order = o;
         }
         else if (k == "buffers") s->scfg.buffers = value != 0.0;

@@ -62,13 +62,16 @@ def test_tiled_odd_tile_rows(mpd):
             _compare_state(sol, o, f"tile_rows={h}")
 
 
-@pytest.mark.parametrize("order,rows", [(1, 1), (1, 3), (1, 8), (2, 1), (2, 2), (2, 5), (2, 8), (2, 37)])
+@pytest.mark.parametrize("order,rows", [(1, 1), (1, 3), (1, 8), (2, 1), (2, 2), (2, 5), (2, 8), (2, 37), (3, 1),
+                                        (3, 3), (3, 8), (3, 37)])
 def test_tiled_xcd_band_order(mpd, order, rows):
     """Tile order 1 (whole row bands per XCD, workgroups past the last chunk
     or strip idle) and order 2 (four vertically adjacent tiles per workgroup,
     alternate waves marching up) give the same bits, including chunk counts
     that are not a multiple of 4 or 8, strip counts that are not a multiple
-    of 4, and a single chunk (three of the four waves idle)."""
+    of 4, and a single chunk (three of the four waves idle); order 3 (a
+    group of strips per XCD, chunk by chunk; 5 strips here, so three XCDs
+    have none)."""
     text = cases.channel_config(37, 300, geometry="condiv", noise=1e-3, max_steps=9, tile_rows=rows)
     print(f"tile_order={order} tile_rows={rows}", flush=True)
     o, _ = run_oracle(text, 0)
