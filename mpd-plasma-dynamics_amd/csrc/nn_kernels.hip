@@ -287,33 +287,59 @@ __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
 // matrix instructions of tile j and stored after them, into the other
 // buffer, so one barrier per k-tile separates the two.  Lane l of an MFMA
 // holds row l % 16 and the 8 k values 8 (l / 16) .. + 7 of each operand;
-// its results are C[4 (l / 16) + i][l % 16], i < 4.
+// its results are C[4 (l / 16) + i][l % 16], i < 4.  An operand whose
+// source is contiguous along k is staged row by row (a tile row holds its 32
+// k values) and read back as one 16-byte run per lane; one contiguous along
+// its rows is staged in source order (a tile row holds one k value for the
+// T rows) and read with gfx950's LDS transpose read (below).  Both stage
+// every 16-byte run with one 16-byte store.
 typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
-constexpr int BK2 = 32, LDK2 = BK2 + 8;  // LDS row pitch (bf16): 80 bytes
+constexpr int BK2 = 32, LDK2 = BK2 + 8;  // LDS row pitch (bf16) of a k-contiguous tile: 80 bytes
 
-// Run r of a T x 32 operand tile: along k, row r / 4, columns 8 (r % 4) ..
-// + 7; along rows, rows 8 (r % (T / 8)) .. + 7 and column r / (T / 8).
-template <int T, bool kRowsFast, class Load>
-__device__ __forceinline__ uint4 stage_load2(int r, int r0, int k0, int k_end, Load ld) {
-    if (kRowsFast) {
-        const int k = k0 + r / (T / 8);
-        return (k < k_end) ? ld(r0 + 8 * (r % (T / 8)), k) : zero_run();
-    }
-    const int k = k0 + 8 * (r & 3);
-    return (k < k_end) ? ld(r0 + (r >> 2), k) : zero_run();
+__device__ __forceinline__ bf16x4 tr_read(const uint16_t* p) {
+    typedef __attribute__((address_space(3))) bf16x4 lds_v4;
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4*)p);
 }
 
+__device__ __forceinline__ bf16x8v join8(bf16x4 lo, bf16x4 hi) {
+    return __builtin_bit_cast(bf16x8v, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
+// Row pitch (bf16) of a tile staged in source order (32 k rows of T
+// values): half the pitch in 4-byte words is 8 or 24 mod 32, so the four k
+// rows one 16-lane group of a transpose read touches (8 words each) fall on
+// distinct banks.
+template <int T>
+__host__ __device__ constexpr int kt_pitch() { return T <= 32 ? 48 : T + 16; }
+template <int T, bool kRowsFast>
+__host__ __device__ constexpr int tile_elems() { return kRowsFast ? BK2 * kt_pitch<T>() : T * LDK2; }
+
+// Run r of a T x 32 operand tile: along k, tile row r / 4, k columns
+// 8 (r % 4) .. + 7; along rows, rows 8 (r % (T / 8)) .. + 7 of k column
+// r / (T / 8), stored at that k's row of the source-order tile.
 template <int T, bool kRowsFast>
 __device__ __forceinline__ void stage_store2(int r, uint16_t* S, uint4 v) {
-    if (kRowsFast) {
-        Run8 u;
-        u.v = v;
-        const int c = r / (T / 8), row = 8 * (r % (T / 8));
-#pragma unroll
-        for (int j = 0; j < 8; ++j) S[(row + j) * LDK2 + c] = u.h[j];
-    } else {
+    if (kRowsFast)
+        *reinterpret_cast<uint4*>(&S[(r / (T / 8)) * kt_pitch<T>() + 8 * (r % (T / 8))]) = v;
+    else
         *reinterpret_cast<uint4*>(&S[(r >> 2) * LDK2 + 8 * (r & 3)]) = v;
+}
+
+// The MFMA operand of rows rb .. rb + 15 of a staged tile for this lane
+// (row l % 16, k values 8 (l / 16) .. + 7).  Source-order tile: in each
+// 16-lane group g, lane s reads k row 8 g + s / 4 (then + 4) at rows
+// rb + 4 (s % 4) .. + 3, and the transpose read hands output lane 4 a + b
+// element b of the reads of lanes a, a + 4, a + 8, a + 12: k rows
+// 8 g .. 8 g + 3 (then + 4 .. + 7) of row rb + 4 a + b.
+template <int T, bool kRowsFast>
+__device__ __forceinline__ bf16x8v frag2(const uint16_t* S, int rb, int lane) {
+    const int s = lane & 15, g = lane >> 4;
+    if (kRowsFast) {
+        constexpr int P = kt_pitch<T>();
+        const uint16_t* p0 = S + (8 * g + (s >> 2)) * P + rb + 4 * (s & 3);
+        return join8(tr_read(p0), tr_read(p0 + 4 * P));
     }
+    return __builtin_bit_cast(bf16x8v, *reinterpret_cast<const uint4*>(&S[(rb + s) * LDK2 + 8 * g]));
 }
 
 // Index k = tap * C + c of a run along a (tap, channel) axis, advanced tile
@@ -368,8 +394,8 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
     constexpr int FM = TM / WM / 16, FN = TN / WN / 16;  // 16 x 16 blocks per wave
     constexpr int RA = (TM * 4 + 255) / 256, RB = (TN * 4 + 255) / 256;  // staging runs per thread
     constexpr bool AR = a_rows_fast<MODE>(), BR = b_rows_fast<MODE>();
-    __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * LDK2];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][TN * LDK2];
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][tile_elems<TM, AR>()];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][tile_elems<TN, BR>()];
     const int t = (int)threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave / WN, wn = wave % WN;
@@ -473,7 +499,7 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
             if (r < TN * 4) stage_store2<TN, BR>(r, Bs[buf], rb[q]);
         }
     };
-    const int row = lane & 15, kq = (lane >> 4) * 8;
+    const int row = lane & 15;
     int buf = 0;
     if (k_begin < k_end) {
         fetch(k_begin);
@@ -485,13 +511,9 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
         if (more) fetch(k0 + BK2);
         bf16x8v fa[FM], fb[FN];
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
-            fa[i] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const uint4*>(
-                                                    &As[buf][((wm * FM + i) * 16 + row) * LDK2 + kq]));
+        for (int i = 0; i < FM; ++i) fa[i] = frag2<TM, AR>(As[buf], (wm * FM + i) * 16, lane);
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
-            fb[j] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const uint4*>(
-                                                    &Bs[buf][((wn * FN + j) * 16 + row) * LDK2 + kq]));
+        for (int j = 0; j < FN; ++j) fb[j] = frag2<TN, BR>(Bs[buf], (wn * FN + j) * 16, lane);
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -552,15 +574,6 @@ struct WgArgs {
 constexpr int WG_CI = 16, WG_XP = WG_CI + 8;  // ci per workgroup; x tile row pitch (bf16, 48 bytes)
 constexpr int WG_RUNS = 8;                     // staging runs per thread per tile, at most
 __host__ __device__ constexpr int wg_dp(int co_t) { return co_t + 8; }  // dz tile row pitch
-
-__device__ __forceinline__ bf16x4 tr_read(const uint16_t* p) {
-    typedef __attribute__((address_space(3))) bf16x4 lds_v4;
-    return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4*)p);
-}
-
-__device__ __forceinline__ bf16x8v join8(bf16x4 lo, bf16x4 hi) {
-    return __builtin_bit_cast(bf16x8v, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
-}
 
 template <int CO_T>
 __global__ void __launch_bounds__(256) wgrad_lds(WgArgs a) {
