@@ -951,6 +951,24 @@ Tile pick_tile(int M, int N) {
     if (t.tm == 16 && t.tn < 64) t.tn = 64;  // four waves need 64 lanes of N then
     return t;
 }
+// The forward and data-gradient tiles: the widest tile, halved (rows
+// first) down to 32 x 64 until about 1024 workgroups run, four per CU: the
+// coarse levels' GEMMs (8400 pixels x 256 channels) would otherwise leave
+// most CUs idle, one 128 x 128 tile per 64 k values of latency each.
+Tile pick_tile_fill(int M, int N) {
+    Tile t = pick_tile(M, N);
+    auto count = [&](const Tile& u) { return (long long)((M + u.tm - 1) / u.tm) * ((N + u.tn - 1) / u.tn); };
+    // Rows halve while at least 32 and at least the tile's width (the
+    // instantiated shapes below have 64 rows or more when 16 wide).
+    auto rows_ok = [&](const Tile& u) { return u.tm > 32 && (u.tn >= 32 || u.tm > 64); };
+    while (count(t) < 1024) {
+        if (rows_ok(t) && t.tm >= t.tn) t.tm /= 2;
+        else if (t.tn > 64) t.tn /= 2;
+        else if (rows_ok(t)) t.tm /= 2;
+        else break;
+    }
+    return t;
+}
 bool vec_ok(int Cin, int Cout) { return Cin % 8 == 0 && Cout % 8 == 0; }
 
 // Pixel splits of the weight gradient: enough workgroups to fill the GPU
@@ -1004,10 +1022,11 @@ bool wgrad_lds_plan(int B, int H, int W, int Cin, int Cout, WgPlan& p) {
     return true;
 }
 
-// Launch of conv_gemm2 for C = a.M x a.N with the tile pick_tile chooses.
+// Launch of conv_gemm2 for C = a.M x a.N with the tile pick_tile (weight
+// gradient: its splits are sized to that tile) or pick_tile_fill chooses.
 template <int MODE>
 void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
-    const Tile t = pick_tile(a.M, a.N);
+    const Tile t = MODE == MODE_WGRAD ? pick_tile(a.M, a.N) : pick_tile_fill(a.M, a.N);
     const dim3 g((unsigned)((a.M + t.tm - 1) / t.tm), (unsigned)((a.N + t.tn - 1) / t.tn), (unsigned)splits);
 #define MPD_GEMM2(TM, TN, WM, WN) \
     if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm2<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return; }
