@@ -41,6 +41,10 @@ int mpd_nn_synchronize(void);
  * be NULL).  x [B][H][W][Cin] bf16, z [B][H][W][Cout] FP32. */
 int mpd_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, float* z, int B, int H, int W,
                        int Cin, int Cout, void* stream);
+/* The same, also writing a = bf16(silu(z)) [B][H][W][Cout] (as
+ * mpd_nn_silu_fwd on z, bit for bit) from the GEMM's registers. */
+int mpd_nn_conv3x3_fwd_silu(const uint16_t* x, const uint16_t* w, const float* bias, float* z, uint16_t* a, int B,
+                            int H, int W, int Cin, int Cout, void* stream);
 /* dx = dL/dx [B][H][W][Cin] FP32 from dz = dL/dz [B][H][W][Cout] bf16. */
 int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, int B, int H, int W, int Cin, int Cout,
                             void* stream);
@@ -59,6 +63,12 @@ int mpd_nn_bias_grad(const float* dz, float* db, float* work, int P, int C, void
 int mpd_nn_silu_fwd(const float* z, uint16_t* a, long long n, void* stream);
 int mpd_nn_silu_bwd(const float* z, const float* da, uint16_t* dz_bf16, float* dz_f32, long long n, void* stream);
 int mpd_nn_to_bf16(const float* x, uint16_t* y, long long n, void* stream);
+/* dz = da silu'(z) ([P][C]; dz = da when z is NULL, a layer without
+ * activation) written as bf16, with db = the bias gradient, as
+ * mpd_nn_bias_grad on the FP32 dz bit for bit, in one pass over z and da;
+ * `work` holds mpd_nn_bias_work_floats(P, C) floats. */
+int mpd_nn_silu_bwd_bias(const float* z, const float* da, uint16_t* dz_bf16, float* db, float* work, int P, int C,
+                         void* stream);
 
 /* 2x2 average pooling (H, W even) and nearest-neighbour x2 upsampling
  * (H, W: the input sizes); backward passes on FP32 gradients. */
@@ -73,6 +83,18 @@ int mpd_nn_concat_fwd(const uint16_t* a, int Ca, const uint16_t* b, int Cb, uint
 int mpd_nn_concat_bwd(const float* dy, float* da, int Ca, int acc_a, float* db, int Cb, int acc_b, long long P,
                       void* stream);
 int mpd_nn_add(float* y, const float* x, long long n, void* stream);
+
+/* The decoder's skip joins in one pass each (H, W: the coarse sizes; channel
+ * counts multiples of 8 forward, of 4 backward; bit for bit the separate
+ * passes): y [B][2H][2W][Cu + Cs] = [upsample2(x) (Cu), skip (Cs)];
+ * backward dx [B][H][W][Cu] = upsample2 backward of dy[..., :Cu] and dskip
+ * [B][2H][2W][Cs] = dy[..., Cu:] (overwritten). */
+int mpd_nn_upcat_fwd(const uint16_t* x, int Cu, const uint16_t* skip, int Cs, uint16_t* y, int B, int H, int W,
+                     void* stream);
+int mpd_nn_upcat_bwd(const float* dy, float* dx, int Cu, float* dskip, int Cs, int B, int H, int W, void* stream);
+/* dx = avgpool2 backward of dy + skip (H, W: the fine sizes, even; C a
+ * multiple of 4), as mpd_nn_avgpool2_bwd then mpd_nn_add. */
+int mpd_nn_pool_bwd_add(const float* dy, const float* skip, float* dx, int B, int H, int W, int C, void* stream);
 
 /* Mean squared error of y against t over n values: dy = 2 (y - t) / n (dy
  * may be NULL) and mpd_nn_mse_blocks(n) partial sums of (y - t)^2 in part. */

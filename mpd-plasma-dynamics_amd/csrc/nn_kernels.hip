@@ -42,6 +42,17 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+__device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
+
+// SiLU's output as bf16, and its backward dz = da silu'(z), silu'(z) =
+// s (1 + z (1 - s)) with s = sigmoid(z): one definition for the separate
+// element-wise kernels and the ones fused into other passes.
+__device__ __forceinline__ uint16_t silu_bf16(float v) { return f2bf(v * sigmoid(v)); }
+__device__ __forceinline__ float silu_grad(float v, float da) {
+    const float s = sigmoid(v);
+    return da * (s * (1.f + v * (1.f - s)));
+}
+
 enum { MODE_FWD = 0, MODE_DGRAD = 1, MODE_WGRAD = 2 };
 
 struct ConvArgs {
@@ -50,6 +61,7 @@ struct ConvArgs {
     const uint16_t* dz;  // pre-activation gradient [B][H][W][Cout]
     const float* bias;   // [Cout] (forward)
     float* out;          // forward: z [P][Cout]; data grad: dx [P][Cin]; weight grad: partials [S][Cout][9 Cin]
+    uint16_t* act;       // forward: bf16(silu(z)) [P][Cout] as well, or null
     int B, H, W, Cin, Cout;
     int M, N, K;
     int kchunk;  // weight grad: pixels per split
@@ -273,6 +285,7 @@ __global__ void __launch_bounds__(256) conv_gemm(ConvArgs a) {
                     float v = acc[i][j][jj];
                     if (MODE == MODE_FWD && a.bias) v += a.bias[n];
                     out[(int64_t)m * a.N + n] = v;
+                    if (MODE == MODE_FWD && a.act) a.act[(int64_t)m * a.N + n] = silu_bf16(v);
                 }
             }
 }
@@ -538,6 +551,7 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
                     float v = acc[i][j][jj];
                     if (MODE == MODE_FWD && a.bias) v += a.bias[n];
                     out[(int64_t)m * a.N + n] = v;
+                    if (MODE == MODE_FWD && a.act) a.act[(int64_t)m * a.N + n] = silu_bf16(v);
                 }
             }
 }
@@ -695,15 +709,22 @@ __global__ void __launch_bounds__(256) wgrad_lds(WgArgs a) {
 }
 
 // out[i] = sum over rows s < S of part[s][i], i < n: a block of 16 waves
-// per 64 outputs, wave w summing rows w, w + 16, ..., then the 16 partial
-// sums added in a fixed tree order (deterministic, whatever the hardware
-// schedule).
+// per 64 outputs, wave w summing rows w, w + 16, ... into 8 running sums
+// (row w + 16 j + 128 k into sum j: eight independent loads in flight),
+// then its 8 sums and the 16 waves' results added in fixed tree orders
+// (deterministic, whatever the hardware schedule).
 __global__ void __launch_bounds__(1024) column_sum(const float* part, float* out, int64_t n, int S) {
     const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
     const int w = (int)(threadIdx.x >> 6);
-    float s = 0.f;
-    if (i < n)
-        for (int z = w; z < S; z += 16) s += part[(int64_t)z * n + i];
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (i < n) {
+        int z = w;
+        for (; z + 112 < S; z += 128)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += part[(int64_t)(z + 16 * j) * n + i];
+        for (int j = 0; z < S; z += 16, ++j) acc[j] += part[(int64_t)z * n + i];
+    }
+    const float s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     __shared__ float sh[16][64];
     sh[w][threadIdx.x & 63] = s;
     __syncthreads();
@@ -714,44 +735,67 @@ __global__ void __launch_bounds__(1024) column_sum(const float* part, float* out
     if (w == 0 && i < n) out[i] = sh[0][threadIdx.x];
 }
 
-// Bias gradient, pass 1: block (cb, s) sums channels 64 cb .. 64 cb + 63
-// over pixel slice s; the 4 waves take interleaved pixels, lane = channel,
-// so each wave reads 64 consecutive floats per pixel.  part[s][c].
-__global__ void __launch_bounds__(256) bias_grad_part(const float* dz, float* part, int P, int C, int rows) {
-    const int c = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
-    const int w = (int)(threadIdx.x >> 6);
+// Per-channel pixel sums (the bias gradient), pass 1: block (cb, s) sums
+// channels cb CW .. cb CW + CW - 1 over pixel slice s.  A wave row holds CW
+// channels of 64 / CW consecutive pixels (CW = 64 when C is a multiple of
+// 64, C when C divides 64, else 64 with the excess lanes idle), so every
+// wave reads 64 consecutive values per step; partial sum q (pixels p0 + q,
+// p0 + q + 4 PL, ...) of a channel sits at LDS slot q CW + channel and the
+// 4 PL partials are added in a fixed tree order (deterministic).  Elem(p, c)
+// gives the value summed; it may also write per-element outputs.
+__host__ __device__ inline int bias_cw(int C) { return C % 64 == 0 ? 64 : (64 % C == 0 ? C : 64); }
+
+template <class Elem>
+__device__ __forceinline__ void channel_slice_sum(float* part, int P, int C, int rows, Elem elem) {
+#pragma clang fp contract(off)  // each value rounded before it is added, as when it is read from memory
+    const int CW = bias_cw(C), PL = 64 / CW;
+    const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
+    const int cc = lane % CW, q = w * PL + lane / CW;
+    const int c = (int)blockIdx.x * CW + cc;
     const int p0 = (int)blockIdx.y * rows, p1 = min(P, p0 + rows);
     float s = 0.f;
     if (c < C)
-        for (int p = p0 + w; p < p1; p += 4) s += dz[(int64_t)p * C + c];
-    __shared__ float sh[4][64];
-    sh[w][threadIdx.x & 63] = s;
+        for (int p = p0 + q; p < p1; p += 4 * PL) s += elem((int64_t)p * C + c);
+    __shared__ float sh[256];
+    sh[t] = s;  // slot q CW + cc
     __syncthreads();
-    if (w == 0 && c < C) part[(int64_t)blockIdx.y * C + c] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) +
-                                                             (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+    for (int h = 2 * PL; h > 0; h >>= 1) {
+        if (q < h) sh[q * CW + cc] += sh[(q + h) * CW + cc];
+        __syncthreads();
+    }
+    if (q == 0 && c < C) part[(int64_t)blockIdx.y * C + c] = sh[cc];
 }
 
-__device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
+__global__ void __launch_bounds__(256) bias_grad_part(const float* dz, float* part, int P, int C, int rows) {
+    channel_slice_sum(part, P, C, rows, [&](int64_t i) { return dz[i]; });
+}
 
 #define GRID_STRIDE(i, n) \
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
 __global__ void silu_fwd(const float* z, uint16_t* a, int64_t n) {
-    GRID_STRIDE(i, n) {
-        const float v = z[i];
-        a[i] = f2bf(v * sigmoid(v));
-    }
+    GRID_STRIDE(i, n) a[i] = silu_bf16(z[i]);
 }
 
 // dz = da * silu'(z); written as bf16 (matrix-core operand) and FP32 (bias).
 __global__ void silu_bwd(const float* z, const float* da, uint16_t* dzb, float* dzf, int64_t n) {
     GRID_STRIDE(i, n) {
-        const float v = z[i];
-        const float s = sigmoid(v);
-        const float d = da[i] * (s * (1.f + v * (1.f - s)));
+        const float d = silu_grad(z[i], da[i]);
         if (dzf) dzf[i] = d;
         dzb[i] = f2bf(d);
     }
+}
+
+// The two fused: dz = da * silu'(z) (dz = da when z is null, a layer
+// without activation) written as bf16, and its per-channel sums over pixel
+// slice blockIdx.y into part, in channel_slice_sum's order.
+__global__ void __launch_bounds__(256) silu_bwd_bias_part(const float* z, const float* da, uint16_t* dzb,
+                                                          float* part, int P, int C, int rows) {
+    channel_slice_sum(part, P, C, rows, [&](int64_t i) {
+        const float d = z ? silu_grad(z[i], da[i]) : da[i];
+        dzb[i] = f2bf(d);
+        return d;
+    });
 }
 
 __global__ void to_bf16(const float* x, uint16_t* y, int64_t n) {
@@ -850,6 +894,75 @@ __global__ void concat_bwd(const float* dy, float* da, int Ca, int acc_a, float*
 
 __global__ void add_inplace(float* y, const float* x, int64_t n) {
     GRID_STRIDE(i, n) y[i] += x[i];
+}
+
+// The decoder's input in one pass: y[b][Y][X] = [x[b][Y/2][X/2] (Cu
+// channels), skip[b][Y][X] (Cs channels)] over the 2H x 2W fine grid, as
+// upsample2_fwd then concat_fwd.  One thread per 16-byte run of y (channel
+// counts multiples of 8), 32-bit indices (the host checks the run count).
+__global__ void upcat_fwd(const uint16_t* x, int Cu, const uint16_t* skip, int Cs, uint16_t* y, int B, int H, int W) {
+    const int Ho = 2 * H, Wo = 2 * W, R = (Cu + Cs) / 8;
+    const int n = B * Ho * Wo * R;
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int)(gridDim.x * blockDim.x)) {
+        const int p = i / R, j = i - p * R;
+        const int X = p % Wo, r = p / Wo, Y = r % Ho, b = r / Ho;
+        const int c = 8 * j;
+        const uint4 v = (c < Cu) ? *reinterpret_cast<const uint4*>(x + ((int64_t)(b * H + Y / 2) * W + X / 2) * Cu + c)
+                                 : *reinterpret_cast<const uint4*>(skip + (int64_t)p * Cs + (c - Cu));
+        *reinterpret_cast<uint4*>(y + (int64_t)p * (Cu + Cs) + c) = v;
+    }
+}
+
+// Its backward in one pass: from dy [B][2H][2W][Cu + Cs], dx[b][y][x][c] =
+// the sum of dy over the 2 x 2 fine pixels of (y, x) at channel c < Cu (in
+// upsample2_bwd's order) and dskip[fine pixel][c'] = dy[...][Cu + c'], as
+// concat_bwd then upsample2_bwd.  One thread per coarse pixel and 4-float
+// run (channel counts multiples of 4), 32-bit indices.
+__global__ void upcat_bwd(const float* dy, float* dx, int Cu, float* dskip, int Cs, int B, int H, int W) {
+    const int Wo = 2 * W, C = Cu + Cs, R = C / 4;
+    const int n = B * H * W * R;
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int)(gridDim.x * blockDim.x)) {
+        const int q = i / R, j = i - q * R;
+        const int x = q % W, r = q / W, y = r % H, b = r / H;
+        const int c = 4 * j;
+        const int64_t f0 = (int64_t)(b * 2 * H + 2 * y) * Wo + 2 * x;  // fine pixels f0, f0 + 1, f0 + Wo, f0 + Wo + 1
+        const int64_t f[4] = {f0, f0 + 1, f0 + Wo, f0 + Wo + 1};
+        float4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = *reinterpret_cast<const float4*>(dy + f[k] * C + c);
+        if (c < Cu) {
+            float4 o;
+            o.x = v[0].x + v[1].x + v[2].x + v[3].x;
+            o.y = v[0].y + v[1].y + v[2].y + v[3].y;
+            o.z = v[0].z + v[1].z + v[2].z + v[3].z;
+            o.w = v[0].w + v[1].w + v[2].w + v[3].w;
+            *reinterpret_cast<float4*>(dx + (int64_t)q * Cu + c) = o;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) *reinterpret_cast<float4*>(dskip + f[k] * Cs + (c - Cu)) = v[k];
+        }
+    }
+}
+
+// dx = avgpool2 backward of dy (H, W the fine sizes) plus skip, as
+// avgpool2_bwd then add_inplace: dx[b][Y][X][c] = 0.25 dy[b][Y/2][X/2][c] +
+// skip[b][Y][X][c].  One thread per 4-float run (C a multiple of 4).
+__global__ void pool_bwd_add(const float* dy, const float* skip, float* dx, int B, int H, int W, int C) {
+#pragma clang fp contract(off)  // the product rounded before the sum, as in the separate passes
+    const int Ho = H / 2, Wo = W / 2, R = C / 4;
+    const int n = B * H * W * R;
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int)(gridDim.x * blockDim.x)) {
+        const int p = i / R, j = i - p * R;
+        const int X = p % W, r = p / W, Y = r % H, b = r / H;
+        const float4 g = *reinterpret_cast<const float4*>(dy + ((int64_t)(b * Ho + Y / 2) * Wo + X / 2) * C + 4 * j);
+        const float4 k = *reinterpret_cast<const float4*>(skip + (int64_t)p * C + 4 * j);
+        float4 o;
+        o.x = 0.25f * g.x + k.x;
+        o.y = 0.25f * g.y + k.y;
+        o.z = 0.25f * g.z + k.z;
+        o.w = 0.25f * g.w + k.w;
+        *reinterpret_cast<float4*>(dx + (int64_t)p * C + 4 * j) = o;
+    }
 }
 
 // Mean squared error over n values: dy = 2 (y - t) / n; per-block partial
@@ -1049,6 +1162,25 @@ void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
     hipLaunchKernelGGL((conv_gemm2<MODE, 128, 128, 2, 2>), g2, dim3(256), 0, s, a);
 }
 
+// The forward pass, z = conv(x, w) + bias, with bf16(silu(z)) written to
+// act as well when act is not null (in the GEMM's epilogue).
+void launch_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, float* z, uint16_t* act, int B, int H,
+                     int W, int Cin, int Cout, hipStream_t s) {
+    ConvArgs a{};
+    a.x = x;
+    a.w = w;
+    a.bias = bias;
+    a.out = z;
+    a.act = act;
+    a.B = B, a.H = H, a.W = W, a.Cin = Cin, a.Cout = Cout;
+    a.M = B * H * W;
+    a.N = Cout;
+    a.K = 9 * Cin;
+    const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
+    if (vec_ok(Cin, Cout)) launch_gemm2<MODE_FWD>(a, 1, s);
+    else hipLaunchKernelGGL((conv_gemm<MODE_FWD, false>), g, dim3(256), 0, s, a);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1083,18 +1215,14 @@ int mpd_nn_synchronize(void) { return finish(hipDeviceSynchronize()); }
 int mpd_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, float* z, int B, int H, int W,
                        int Cin, int Cout, void* stream) {
     if (!x || !w || !z || !conv_dims_ok(B, H, W, Cin, Cout)) return bad("conv3x3_fwd: bad argument");
-    ConvArgs a{};
-    a.x = x;
-    a.w = w;
-    a.bias = bias;
-    a.out = z;
-    a.B = B, a.H = H, a.W = W, a.Cin = Cin, a.Cout = Cout;
-    a.M = B * H * W;
-    a.N = Cout;
-    a.K = 9 * Cin;
-    const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
-    if (vec_ok(Cin, Cout)) launch_gemm2<MODE_FWD>(a, 1, (hipStream_t)stream);
-    else hipLaunchKernelGGL((conv_gemm<MODE_FWD, false>), g, dim3(256), 0, (hipStream_t)stream, a);
+    launch_conv_fwd(x, w, bias, z, nullptr, B, H, W, Cin, Cout, (hipStream_t)stream);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_conv3x3_fwd_silu(const uint16_t* x, const uint16_t* w, const float* bias, float* z, uint16_t* a, int B,
+                            int H, int W, int Cin, int Cout, void* stream) {
+    if (!x || !w || !z || !a || !conv_dims_ok(B, H, W, Cin, Cout)) return bad("conv3x3_fwd_silu: bad argument");
+    launch_conv_fwd(x, w, bias, z, a, B, H, W, Cin, Cout, (hipStream_t)stream);
     return finish(hipSuccess);
 }
 
@@ -1162,8 +1290,20 @@ int mpd_nn_bias_grad(const float* dz, float* db, float* work, int P, int C, void
     const int S = bias_slices(P, C);
     const int rows = (P + S - 1) / S;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(bias_grad_part, dim3((unsigned)((C + 63) / 64), (unsigned)S), dim3(256), 0, s, dz, work, P, C,
-                       rows);
+    hipLaunchKernelGGL(bias_grad_part, dim3((unsigned)((C + bias_cw(C) - 1) / bias_cw(C)), (unsigned)S), dim3(256), 0,
+                       s, dz, work, P, C, rows);
+    hipLaunchKernelGGL(column_sum, dim3((unsigned)((C + 63) / 64)), dim3(1024), 0, s, (const float*)work, db, (int64_t)C, S);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_silu_bwd_bias(const float* z, const float* da, uint16_t* dz_bf16, float* db, float* work, int P, int C,
+                         void* stream) {
+    if (!da || !dz_bf16 || !db || !work || P <= 0 || C <= 0) return bad("silu_bwd_bias: bad argument");
+    const int S = bias_slices(P, C);
+    const int rows = (P + S - 1) / S;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(silu_bwd_bias_part, dim3((unsigned)((C + bias_cw(C) - 1) / bias_cw(C)), (unsigned)S), dim3(256),
+                       0, s, z, da, dz_bf16, work, P, C, rows);
     hipLaunchKernelGGL(column_sum, dim3((unsigned)((C + 63) / 64)), dim3(1024), 0, s, (const float*)work, db, (int64_t)C, S);
     return finish(hipSuccess);
 }
@@ -1232,6 +1372,37 @@ int mpd_nn_concat_bwd(const float* dy, float* da, int Ca, int acc_a, float* db, 
     if (!dy || !da || !db || Ca <= 0 || Cb <= 0 || P < 0) return bad("concat_bwd: bad argument");
     hipLaunchKernelGGL(concat_bwd, dim3(blocks_for(P * (Ca + Cb))), dim3(256), 0, (hipStream_t)stream, dy, da, Ca,
                        acc_a, db, Cb, acc_b, (int64_t)P);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_upcat_fwd(const uint16_t* x, int Cu, const uint16_t* skip, int Cs, uint16_t* y, int B, int H, int W,
+                     void* stream) {
+    const long long runs = 4LL * B * H * W * ((Cu + Cs) / 8);
+    if (!x || !skip || !y || B <= 0 || H <= 0 || W <= 0 || Cu <= 0 || Cs <= 0 || Cu % 8 || Cs % 8 ||
+        runs >= (1LL << 31))
+        return bad("upcat_fwd: bad argument (channel counts must be positive multiples of 8)");
+    hipLaunchKernelGGL(upcat_fwd, dim3(blocks_for(runs)), dim3(256), 0, (hipStream_t)stream, x, Cu, skip, Cs, y, B, H,
+                       W);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_upcat_bwd(const float* dy, float* dx, int Cu, float* dskip, int Cs, int B, int H, int W, void* stream) {
+    const long long runs = 1LL * B * H * W * ((Cu + Cs) / 4);
+    if (!dy || !dx || !dskip || B <= 0 || H <= 0 || W <= 0 || Cu <= 0 || Cs <= 0 || Cu % 4 || Cs % 4 ||
+        4LL * B * H * W * (Cu + Cs) >= (1LL << 31))
+        return bad("upcat_bwd: bad argument (channel counts must be positive multiples of 4)");
+    hipLaunchKernelGGL(upcat_bwd, dim3(blocks_for(runs)), dim3(256), 0, (hipStream_t)stream, dy, dx, Cu, dskip, Cs, B,
+                       H, W);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_pool_bwd_add(const float* dy, const float* skip, float* dx, int B, int H, int W, int C, void* stream) {
+    const long long runs = 1LL * B * H * W * (C / 4);
+    if (!dy || !skip || !dx || B <= 0 || H <= 0 || W <= 0 || H % 2 || W % 2 || C <= 0 || C % 4 ||
+        1LL * B * H * W * C >= (1LL << 31))
+        return bad("pool_bwd_add: bad argument (H, W even, C a positive multiple of 4)");
+    hipLaunchKernelGGL(pool_bwd_add, dim3(blocks_for(runs)), dim3(256), 0, (hipStream_t)stream, dy, skip, dx, B, H, W,
+                       C);
     return finish(hipSuccess);
 }
 

@@ -52,6 +52,7 @@ def nn_lib():
     i, ll, f = C.c_int, C.c_longlong, C.c_float
     sig = {
         "mpd_nn_conv3x3_fwd": [_vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
+        "mpd_nn_conv3x3_fwd_silu": [_vp, _vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
         "mpd_nn_conv3x3_bwd_data": [_vp, _vp, _vp, i, i, i, i, i, _vp],
         "mpd_nn_conv3x3_bwd_weight": [_vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
         "mpd_nn_wgrad_work_floats": [i, i, i, i, i],
@@ -59,6 +60,7 @@ def nn_lib():
         "mpd_nn_bias_work_floats": [i, i],
         "mpd_nn_silu_fwd": [_vp, _vp, ll, _vp],
         "mpd_nn_silu_bwd": [_vp, _vp, _vp, _vp, ll, _vp],
+        "mpd_nn_silu_bwd_bias": [_vp, _vp, _vp, _vp, _vp, i, i, _vp],
         "mpd_nn_to_bf16": [_vp, _vp, ll, _vp],
         "mpd_nn_avgpool2_fwd": [_vp, _vp, i, i, i, i, _vp],
         "mpd_nn_avgpool2_bwd": [_vp, _vp, i, i, i, i, _vp],
@@ -67,6 +69,9 @@ def nn_lib():
         "mpd_nn_concat_fwd": [_vp, i, _vp, i, _vp, ll, _vp],
         "mpd_nn_concat_bwd": [_vp, _vp, i, i, _vp, i, i, ll, _vp],
         "mpd_nn_add": [_vp, _vp, ll, _vp],
+        "mpd_nn_upcat_fwd": [_vp, i, _vp, i, _vp, i, i, i, _vp],
+        "mpd_nn_upcat_bwd": [_vp, _vp, i, _vp, i, i, i, i, _vp],
+        "mpd_nn_pool_bwd_add": [_vp, _vp, _vp, i, i, i, i, _vp],
         "mpd_nn_mse_blocks": [ll],
         "mpd_nn_mse": [_vp, _vp, _vp, _vp, ll, _vp],
         "mpd_nn_adam": [_vp, _vp, _vp, _vp, _vp, ll, f, f, f, f, i, f, _vp],
@@ -90,7 +95,9 @@ def nn_lib():
 
 
 # Every symbol include/mpd_nn.h declares.
-NN_EXPORTS = ["mpd_nn_last_error", "mpd_nn_conv3x3_fwd", "mpd_nn_conv3x3_bwd_data", "mpd_nn_wgrad_work_floats",
+NN_EXPORTS = ["mpd_nn_last_error", "mpd_nn_conv3x3_fwd", "mpd_nn_conv3x3_fwd_silu", "mpd_nn_conv3x3_bwd_data",
+              "mpd_nn_wgrad_work_floats", "mpd_nn_silu_bwd_bias", "mpd_nn_upcat_fwd", "mpd_nn_upcat_bwd",
+              "mpd_nn_pool_bwd_add",
               "mpd_nn_conv3x3_bwd_weight", "mpd_nn_bias_work_floats", "mpd_nn_bias_grad", "mpd_nn_silu_fwd", "mpd_nn_silu_bwd",
               "mpd_nn_to_bf16", "mpd_nn_avgpool2_fwd", "mpd_nn_avgpool2_bwd", "mpd_nn_upsample2_fwd",
               "mpd_nn_upsample2_bwd", "mpd_nn_concat_fwd", "mpd_nn_concat_bwd", "mpd_nn_add", "mpd_nn_mse_blocks",
@@ -282,7 +289,7 @@ class UNet:
         ch = [self.base * 2 ** i for i in range(self.levels)]
         k = 0
         self.x_in = DeviceArray((B, H, W, self.cin), "bf16")
-        self.pools, self.ups, self.cats = [], [], []
+        self.pools, self.cats = [], []
         for i in range(self.levels):
             for j in range(2):
                 ci, co = self.layers[k]
@@ -296,7 +303,6 @@ class UNet:
         for i in range(self.levels - 2, -1, -1):
             h, w = h * 2, w * 2
             cu = self.layers[k][0] - ch[i]
-            self.ups.append(DeviceArray((B, h, w, cu), "bf16"))
             self.cats.append(DeviceArray((B, h, w, cu + ch[i]), "bf16"))
             for j in range(2):
                 ci, co = self.layers[k]
@@ -313,13 +319,11 @@ class UNet:
         bwork = max(L.mpd_nn_bias_work_floats(B * d["h"] * d["w"], d["co"]) for d in self.lay)
         self.bwork = DeviceArray(int(bwork))
         big = max(B * d["h"] * d["w"] * max(d["ci"], d["co"]) for d in self.lay)
-        # Gradient scratch: two ping-pong FP32 buffers, bf16 dz, FP32 dz, and
+        # Gradient scratch: two ping-pong FP32 buffers, bf16 dz, and
         # per-skip gradient buffers.
         self.g0 = DeviceArray(big)
         self.g1 = DeviceArray(big)
-        self.g2 = DeviceArray(big)
         self.dzb = DeviceArray(big, "bf16")
-        self.dzf = DeviceArray(big)
         self.dskip = [DeviceArray(B * H // 2 ** i * W // 2 ** i * ch[i])
                       for i in range(self.levels - 1)]
         self.loss_part = DeviceArray(1024)
@@ -338,9 +342,8 @@ class UNet:
             for j in range(2):
                 d = self.lay[k]
                 d["a_in"] = a
-                _ck(L.mpd_nn_conv3x3_fwd(_p(a), _p(self.wb[k]), _p(self.b[k]), _p(d["z"]), B, d["h"], d["w"],
-                                         d["ci"], d["co"], s))
-                _ck(L.mpd_nn_silu_fwd(_p(d["z"]), _p(d["a_out"]), d["z"].numel(), s))
+                _ck(L.mpd_nn_conv3x3_fwd_silu(_p(a), _p(self.wb[k]), _p(self.b[k]), _p(d["z"]), _p(d["a_out"]), B,
+                                              d["h"], d["w"], d["ci"], d["co"], s))
                 a = d["a_out"]
                 k += 1
             if i < self.levels - 1:
@@ -352,16 +355,13 @@ class UNet:
             d = self.lay[k]
             hh, ww = d["h"] // 2, d["w"] // 2
             cu = d["ci"] - skips[i].shape[-1]
-            _ck(L.mpd_nn_upsample2_fwd(_p(a), _p(self.ups[n]), B, hh, ww, cu, s))
-            _ck(L.mpd_nn_concat_fwd(_p(self.ups[n]), cu, _p(skips[i]), skips[i].shape[-1], _p(self.cats[n]),
-                                    B * d["h"] * d["w"], s))
+            _ck(L.mpd_nn_upcat_fwd(_p(a), cu, _p(skips[i]), skips[i].shape[-1], _p(self.cats[n]), B, hh, ww, s))
             a = self.cats[n]
             for j in range(2):
                 d = self.lay[k]
                 d["a_in"] = a
-                _ck(L.mpd_nn_conv3x3_fwd(_p(a), _p(self.wb[k]), _p(self.b[k]), _p(d["z"]), B, d["h"], d["w"],
-                                         d["ci"], d["co"], s))
-                _ck(L.mpd_nn_silu_fwd(_p(d["z"]), _p(d["a_out"]), d["z"].numel(), s))
+                _ck(L.mpd_nn_conv3x3_fwd_silu(_p(a), _p(self.wb[k]), _p(self.b[k]), _p(d["z"]), _p(d["a_out"]), B,
+                                              d["h"], d["w"], d["ci"], d["co"], s))
                 a = d["a_out"]
                 k += 1
         d = self.lay[k]
@@ -376,14 +376,9 @@ class UNet:
         dL/d(input) into dx_out (FP32, or None for the first conv)."""
         L, s = nn_lib(), _stream()
         d = self.lay[k]
-        n = B * d["h"] * d["w"] * d["co"]
-        if d["act"]:
-            _ck(L.mpd_nn_silu_bwd(_p(d["z"]), _p(da), _p(self.dzb), _p(self.dzf), n, s))
-            dzf = self.dzf
-        else:
-            _ck(L.mpd_nn_to_bf16(_p(da), _p(self.dzb), n, s))
-            dzf = da
-        _ck(L.mpd_nn_bias_grad(_p(dzf), _p(self.gb[k]), _p(self.bwork), B * d["h"] * d["w"], d["co"], s))
+        z = _p(d["z"]) if d["act"] else None  # no activation: dz = da
+        _ck(L.mpd_nn_silu_bwd_bias(z, _p(da), _p(self.dzb), _p(self.gb[k]), _p(self.bwork), B * d["h"] * d["w"],
+                                   d["co"], s))
         _ck(L.mpd_nn_conv3x3_bwd_weight(_p(d["a_in"]), _p(self.dzb), _p(self.gw[k]), _p(self.work), B, d["h"],
                                         d["w"], d["ci"], d["co"], s))
         if dx_out is not None:
@@ -394,7 +389,7 @@ class UNet:
         """dy: dL/d(output) FP32 (B, H, W, cout).  Fills the gradients."""
         L, s = nn_lib(), _stream()
         k = len(self.lay) - 1
-        cur, nxt, tmp = self.g0, self.g1, self.g2
+        cur, nxt = self.g0, self.g1
         self._conv_back(k, dy, cur, B)
         k -= 1
         ch = [self.base * 2 ** i for i in range(self.levels)]
@@ -407,17 +402,15 @@ class UNet:
             cur, nxt = nxt, cur
             k -= 1
             cu = d["ci"] - ch[i]
-            P = B * d["h"] * d["w"]
-            _ck(L.mpd_nn_concat_bwd(_p(cur), _p(tmp), cu, 0, _p(self.dskip[i]), ch[i], 0, P, s))
-            _ck(L.mpd_nn_upsample2_bwd(_p(tmp), _p(cur), B, d["h"] // 2, d["w"] // 2, cu, s))
+            _ck(L.mpd_nn_upcat_bwd(_p(cur), _p(nxt), cu, _p(self.dskip[i]), ch[i], B, d["h"] // 2, d["w"] // 2, s))
+            cur, nxt = nxt, cur
         for j in range(2):  # bottleneck
             self._conv_back(k, cur, nxt, B)
             cur, nxt = nxt, cur
             k -= 1
         for i in range(self.levels - 2, -1, -1):  # encoder levels, coarsest first
             d = self.lay[k]
-            _ck(L.mpd_nn_avgpool2_bwd(_p(cur), _p(nxt), B, d["h"], d["w"], d["co"], s))
-            _ck(L.mpd_nn_add(_p(nxt), _p(self.dskip[i]), B * d["h"] * d["w"] * d["co"], s))
+            _ck(L.mpd_nn_pool_bwd_add(_p(cur), _p(self.dskip[i]), _p(nxt), B, d["h"], d["w"], d["co"], s))
             cur, nxt = nxt, cur
             self._conv_back(k, cur, nxt, B)
             cur, nxt = nxt, cur

@@ -165,7 +165,9 @@ def test_elementwise_kernels_match_oracle(nn):
     np.testing.assert_allclose(_host(Wd), rw, rtol=1e-5, atol=1e-6)
     assert np.array_equal(_host_bf16(WB), S.bf16_round(_host(Wd)))
 
-    for P, C in ((5, 3), (3000, 70), (40000, 8)):
+    # Channel counts of every lane layout: 64 wide with idle lanes (3, 70),
+    # C dividing 64 (1, 8, 16, 32), multiples of 64 (128).
+    for P, C in ((5, 3), (3000, 70), (40000, 8), (1000, 1), (9000, 16), (20000, 32), (7000, 128)):
         dzb = rng.normal(size=(P, C)).astype(np.float32)
         DB = _empty((C,))
         work = _empty((int(L.mpd_nn_bias_work_floats(P, C)),))
@@ -186,6 +188,93 @@ def test_elementwise_kernels_match_oracle(nn):
     I = nn.DeviceArray.from_numpy(idx, "i32")
     nn._ck(L.mpd_nn_gather_rows(nn._p(_dev_f32(src)), nn._p(I), nn._p(dst), 3, 4, s))
     assert np.array_equal(_host(dst), src[idx].astype(np.float64))
+
+
+@pytest.mark.parametrize("shape", [(2, 9, 7, 16, 8), (1, 20, 24, 64, 128), (2, 6, 5, 40, 70), (1, 5, 168, 32, 32),
+                                   (2, 13, 21, 64, 96)])
+def test_fused_passes_equal_separate(nn, shape):
+    """The fused entry points equal the separate passes bit for bit: the
+    forward GEMM with SiLU in its epilogue against the forward then silu_fwd,
+    and the SiLU backward with the bias gradient (and without activation,
+    z = NULL) against silu_bwd / to_bf16 then bias_grad."""
+    B, H, W, Ci, Co = shape
+    print(f"fused passes B={B} H={H} W={W} Cin={Ci} Cout={Co}", flush=True)
+    rng = np.random.default_rng(sum(shape) + 1)
+    L, s = nn.nn_lib(), nn._stream()
+    x = S.bf16_round(rng.normal(size=(B, H, W, Ci)))
+    w = S.bf16_round(rng.normal(0, 0.3, size=(Co, 3, 3, Ci)))
+    b = rng.normal(size=Co).astype(np.float32).astype(np.float64)
+    X, Wt, Bt = _dev_bf16(x), _dev_bf16(w), _dev_f32(b)
+    Z1, Z2 = _empty((B, H, W, Co)), _empty((B, H, W, Co))
+    A1, A2 = _empty((B, H, W, Co), bf16=True), _empty((B, H, W, Co), bf16=True)
+    nn._ck(L.mpd_nn_conv3x3_fwd(nn._p(X), nn._p(Wt), nn._p(Bt), nn._p(Z1), B, H, W, Ci, Co, s))
+    nn._ck(L.mpd_nn_silu_fwd(nn._p(Z1), nn._p(A1), Z1.numel(), s))
+    nn._ck(L.mpd_nn_conv3x3_fwd_silu(nn._p(X), nn._p(Wt), nn._p(Bt), nn._p(Z2), nn._p(A2), B, H, W, Ci, Co, s))
+    assert np.array_equal(Z1.download(), Z2.download())
+    assert np.array_equal(A1.download(), A2.download())
+
+    P, n = B * H * W, B * H * W * Co
+    da = rng.normal(size=(B, H, W, Co)).astype(np.float32).astype(np.float64)
+    DA = _dev_f32(da)
+    work = _empty((int(L.mpd_nn_bias_work_floats(P, Co)),))
+    for act in (True, False):
+        DZB1, DZB2, DZF = _empty((n,), bf16=True), _empty((n,), bf16=True), _empty((n,))
+        DB1, DB2 = _empty((Co,)), _empty((Co,))
+        if act:
+            nn._ck(L.mpd_nn_silu_bwd(nn._p(Z1), nn._p(DA), nn._p(DZB1), nn._p(DZF), n, s))
+            nn._ck(L.mpd_nn_bias_grad(nn._p(DZF), nn._p(DB1), nn._p(work), P, Co, s))
+        else:
+            nn._ck(L.mpd_nn_to_bf16(nn._p(DA), nn._p(DZB1), n, s))
+            nn._ck(L.mpd_nn_bias_grad(nn._p(DA), nn._p(DB1), nn._p(work), P, Co, s))
+        nn._ck(L.mpd_nn_silu_bwd_bias(nn._p(Z1) if act else None, nn._p(DA), nn._p(DZB2), nn._p(DB2), nn._p(work), P,
+                                      Co, s))
+        assert np.array_equal(DZB1.download(), DZB2.download()), act
+        assert np.array_equal(DB1.download(), DB2.download()), act
+        if act:
+            ref = (da * S.silu_grad(_host(Z1))).reshape(P, Co)
+            mag = np.abs(ref).sum(axis=0)
+            assert np.all(np.abs(_host(DB2) - ref.sum(axis=0)) <= P * 2.0 ** -22 * mag + 1e-30)
+
+
+@pytest.mark.parametrize("B,H,W,Cu,Cs", [(2, 3, 5, 16, 8), (1, 25, 21, 64, 32), (2, 4, 6, 8, 24), (1, 50, 84, 64, 32)])
+def test_fused_joins_equal_separate(nn, B, H, W, Cu, Cs):
+    """The decoder's skip joins in one pass equal the separate passes bit for
+    bit: upsample + concatenation forward, their backward, and the encoder's
+    pooling backward plus the skip gradient (H, W the coarse sizes)."""
+    print(f"fused joins B={B} H={H} W={W} Cu={Cu} Cs={Cs}", flush=True)
+    rng = np.random.default_rng(B * H * W + Cu + Cs)
+    L, s = nn.nn_lib(), nn._stream()
+    Hf, Wf, P = 2 * H, 2 * W, B * 4 * H * W
+    x = S.bf16_round(rng.normal(size=(B, H, W, Cu)))
+    sk = S.bf16_round(rng.normal(size=(B, Hf, Wf, Cs)))
+    X, SK = _dev_bf16(x), _dev_bf16(sk)
+    U, Y1, Y2 = (_empty((B, Hf, Wf, Cu), bf16=True), _empty((B, Hf, Wf, Cu + Cs), bf16=True),
+                 _empty((B, Hf, Wf, Cu + Cs), bf16=True))
+    nn._ck(L.mpd_nn_upsample2_fwd(nn._p(X), nn._p(U), B, H, W, Cu, s))
+    nn._ck(L.mpd_nn_concat_fwd(nn._p(U), Cu, nn._p(SK), Cs, nn._p(Y1), P, s))
+    nn._ck(L.mpd_nn_upcat_fwd(nn._p(X), Cu, nn._p(SK), Cs, nn._p(Y2), B, H, W, s))
+    assert np.array_equal(Y1.download(), Y2.download())
+    assert np.array_equal(_host_bf16(Y2), np.concatenate([S.up_fwd(x), sk], axis=-1))
+
+    dy = rng.normal(size=(B, Hf, Wf, Cu + Cs)).astype(np.float32)
+    DY = _dev_f32(dy)
+    T, DX1, DS1 = _empty((B, Hf, Wf, Cu)), _empty((B, H, W, Cu)), _empty((B, Hf, Wf, Cs))
+    DX2, DS2 = _empty((B, H, W, Cu)), _empty((B, Hf, Wf, Cs))
+    nn._ck(L.mpd_nn_concat_bwd(nn._p(DY), nn._p(T), Cu, 0, nn._p(DS1), Cs, 0, P, s))
+    nn._ck(L.mpd_nn_upsample2_bwd(nn._p(T), nn._p(DX1), B, H, W, Cu, s))
+    nn._ck(L.mpd_nn_upcat_bwd(nn._p(DY), nn._p(DX2), Cu, nn._p(DS2), Cs, B, H, W, s))
+    assert np.array_equal(DX1.download(), DX2.download())
+    assert np.array_equal(DS1.download(), DS2.download())
+    np.testing.assert_allclose(_host(DX2), S.up_bwd(dy[..., :Cu].astype(np.float64)), rtol=1e-6, atol=1e-6)
+
+    g = rng.normal(size=(B, H, W, Cs)).astype(np.float32)
+    G = _dev_f32(g)
+    PB1, PB2 = _empty((B, Hf, Wf, Cs)), _empty((B, Hf, Wf, Cs))
+    nn._ck(L.mpd_nn_avgpool2_bwd(nn._p(G), nn._p(PB1), B, Hf, Wf, Cs, s))
+    nn._ck(L.mpd_nn_add(nn._p(PB1), nn._p(DS2), B * Hf * Wf * Cs, s))
+    nn._ck(L.mpd_nn_pool_bwd_add(nn._p(G), nn._p(DS2), nn._p(PB2), B, Hf, Wf, Cs, s))
+    assert np.array_equal(PB1.download(), PB2.download())
+    np.testing.assert_allclose(_host(PB2), S.pool_bwd(g.astype(np.float64)) + dy[..., Cu:], rtol=1e-6, atol=1e-7)
 
 
 @pytest.mark.parametrize("base,H,W,B", [(8, 16, 8, 2), (32, 16, 8, 2), (32, 24, 40, 1)])
