@@ -65,7 +65,7 @@ def trained(ws):
     norm = ws.Normalizer(fields - base)
     net = SG.UNet(L, M + 1, seed=3)
     losses = ws.train(net, SG.features(params, radii, L, M), norm.encode(fields - base).astype(np.float32),
-                      epochs=80, batch=16, lr=2e-3, log_every=40)
+                      epochs=200, batch=16, lr=2e-3, log_every=50)
     assert losses[-1] < losses[0] / 50, losses[::10]
     return net, norm
 
