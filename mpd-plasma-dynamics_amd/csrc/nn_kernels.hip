@@ -590,7 +590,7 @@ constexpr int WG_RUNS = 8;                     // staging runs per thread per ti
 __host__ __device__ constexpr int wg_dp(int co_t) { return co_t + 8; }  // dz tile row pitch
 
 template <int CO_T>
-__global__ void __launch_bounds__(256) wgrad_lds(WgArgs a) {
+__global__ void __launch_bounds__(256, 2) wgrad_lds(WgArgs a) {
     constexpr int DP = wg_dp(CO_T), FC = CO_T / 16, NW = 3;  // taps per wave: w, w + 4, w + 8 (< 9)
     extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
     const int W = a.W, W2 = W + 2, RC = a.RC;
@@ -1112,7 +1112,7 @@ size_t wgrad_lds_bytes(int W, int RC, int co_t) {
 }
 bool wgrad_lds_plan(int B, int H, int W, int Cin, int Cout, WgPlan& p) {
     if (Cin % WG_CI != 0 || Cout % 16 != 0) return false;
-    p.co_t = (Cout % 32 == 0) ? 32 : 16;
+    p.co_t = (Cout % 64 == 0) ? 64 : (Cout % 32 == 0) ? 32 : 16;
     constexpr size_t kMaxLds = 60 * 1024;
     int RC = std::max(1, std::min(H, 512 / W));
     // Both staged tiles within 256 WG_RUNS 16-byte runs (the kernel's
@@ -1264,7 +1264,8 @@ int mpd_nn_conv3x3_bwd_weight(const uint16_t* x, const uint16_t* dz, float* dw, 
     if (wgrad_lds_plan(B, H, W, Cin, Cout, pl)) {
         WgArgs w{x, dz, work, B, H, W, Cin, Cout, pl.RC, pl.rps, pl.spi};
         const dim3 g((unsigned)(Cout / pl.co_t), (unsigned)(Cin / WG_CI), (unsigned)pl.S);
-        if (pl.co_t == 32) hipLaunchKernelGGL(wgrad_lds<32>, g, dim3(256), pl.lds, s, w);
+        if (pl.co_t == 64) hipLaunchKernelGGL(wgrad_lds<64>, g, dim3(256), pl.lds, s, w);
+        else if (pl.co_t == 32) hipLaunchKernelGGL(wgrad_lds<32>, g, dim3(256), pl.lds, s, w);
         else hipLaunchKernelGGL(wgrad_lds<16>, g, dim3(256), pl.lds, s, w);
         const int64_t n = (int64_t)Cout * 9 * Cin;
         hipLaunchKernelGGL(column_sum, dim3((unsigned)((n + 63) / 64)), dim3(1024), 0, s, (const float*)work, dw, n,
