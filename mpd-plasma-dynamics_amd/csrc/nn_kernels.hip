@@ -1101,8 +1101,13 @@ int wgrad_chunk(int P, int S) { return ((P + S - 1) / S + BK - 1) / BK * BK; }
 
 // Launch plan of wgrad_lds: channel counts multiples of 16, rows per chunk
 // for about 512 pixels with the LDS tiles within 60 KiB (two workgroups per
-// CU), slices of whole chunks so that about 1024 workgroups run; false when
-// a single row's tiles do not fit (very wide images: the GEMM path then).
+// CU), slices of whole chunks; false when a single row's tiles do not fit
+// (very wide images: the GEMM path then).  The chunks per slice minimize
+// rounds x chunks per slice, a round being the 512 workgroups MI355X holds
+// at once (256 CUs, two each: the kernel's registers and LDS), over slice
+// counts up to about 1024 workgroups (each slice adds a partial sum), ties
+// to fewer slices: 800 workgroups of 4 chunks (two rounds, the second 56 %
+// full) become 464 of 7 (one round).
 struct WgPlan {
     int co_t, RC, rps, spi, S;
     size_t lds;
@@ -1127,8 +1132,20 @@ bool wgrad_lds_plan(int B, int H, int W, int Cin, int Cout, WgPlan& p) {
     p.lds = wgrad_lds_bytes(W, RC, p.co_t);
     const int tiles = (Cout / p.co_t) * (Cin / WG_CI);
     const int chunks = (H + RC - 1) / RC;
+    constexpr long long kSlots = 512;
+    // At most about 1024 workgroups: more slices cost more partial sums.
     const int want = std::max(1, 1024 / (tiles * B));
-    const int cps = (chunks + std::min(chunks, want) - 1) / std::min(chunks, want);
+    const int cps_min = (chunks + std::min(chunks, want) - 1) / std::min(chunks, want);
+    int cps = chunks;
+    long long best = -1;
+    for (int c = chunks; c >= cps_min; --c) {
+        const long long wgs = (long long)tiles * B * ((chunks + c - 1) / c);
+        const long long cost = (wgs + kSlots - 1) / kSlots * c;
+        if (best < 0 || cost < best) {
+            best = cost;
+            cps = c;
+        }
+    }
     p.rps = cps * RC;
     p.spi = (H + p.rps - 1) / p.rps;
     p.S = B * p.spi;
