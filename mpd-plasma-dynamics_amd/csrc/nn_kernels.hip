@@ -741,12 +741,14 @@ __global__ void __launch_bounds__(1024) column_sum(const float* part, float* out
 // 64, C when C divides 64, else 64 with the excess lanes idle), so every
 // wave reads 64 consecutive values per step; partial sum q (pixels p0 + q,
 // p0 + q + 4 PL, ...) of a channel sits at LDS slot q CW + channel and the
-// 4 PL partials are added in a fixed tree order (deterministic).  Elem(p, c)
-// gives the value summed; it may also write per-element outputs.
+// 4 PL partials are added in a fixed tree order (deterministic).  For flat
+// index i = p C + c, ld(i) loads the inputs and val(i, loaded) gives the
+// value summed (it may also write per-element outputs); four pixels' loads
+// are issued before their values are formed.
 __host__ __device__ inline int bias_cw(int C) { return C % 64 == 0 ? 64 : (64 % C == 0 ? C : 64); }
 
-template <class Elem>
-__device__ __forceinline__ void channel_slice_sum(float* part, int P, int C, int rows, Elem elem) {
+template <class Ld, class Val>
+__device__ __forceinline__ void channel_slice_sum(float* part, int P, int C, int rows, Ld ld, Val val) {
 #pragma clang fp contract(off)  // each value rounded before it is added, as when it is read from memory
     const int CW = bias_cw(C), PL = 64 / CW;
     const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
@@ -754,8 +756,24 @@ __device__ __forceinline__ void channel_slice_sum(float* part, int P, int C, int
     const int c = (int)blockIdx.x * CW + cc;
     const int p0 = (int)blockIdx.y * rows, p1 = min(P, p0 + rows);
     float s = 0.f;
-    if (c < C)
-        for (int p = p0 + q; p < p1; p += 4 * PL) s += elem((int64_t)p * C + c);
+    if (c < C) {
+        // Four pixels per pass, their values summed in pixel order.
+        const int step = 4 * PL;
+        int p = p0 + q;
+        for (; p + 3 * step < p1; p += 4 * step) {
+            const int64_t i0 = (int64_t)p * C + c, i1 = i0 + (int64_t)step * C, i2 = i1 + (int64_t)step * C,
+                          i3 = i2 + (int64_t)step * C;
+            const auto t0 = ld(i0), t1 = ld(i1), t2 = ld(i2), t3 = ld(i3);
+            s += val(i0, t0);
+            s += val(i1, t1);
+            s += val(i2, t2);
+            s += val(i3, t3);
+        }
+        for (; p < p1; p += step) {
+            const int64_t i = (int64_t)p * C + c;
+            s += val(i, ld(i));
+        }
+    }
     __shared__ float sh[256];
     sh[t] = s;  // slot q CW + cc
     __syncthreads();
@@ -766,8 +784,9 @@ __device__ __forceinline__ void channel_slice_sum(float* part, int P, int C, int
     if (q == 0 && c < C) part[(int64_t)blockIdx.y * C + c] = sh[cc];
 }
 
-__global__ void __launch_bounds__(256) bias_grad_part(const float* dz, float* part, int P, int C, int rows) {
-    channel_slice_sum(part, P, C, rows, [&](int64_t i) { return dz[i]; });
+__global__ void __launch_bounds__(256) bias_grad_part(const float* __restrict__ dz, float* __restrict__ part, int P,
+                                                      int C, int rows) {
+    channel_slice_sum(part, P, C, rows, [&](int64_t i) { return dz[i]; }, [](int64_t, float v) { return v; });
 }
 
 #define GRID_STRIDE(i, n) \
@@ -789,13 +808,17 @@ __global__ void silu_bwd(const float* z, const float* da, uint16_t* dzb, float* 
 // The two fused: dz = da * silu'(z) (dz = da when z is null, a layer
 // without activation) written as bf16, and its per-channel sums over pixel
 // slice blockIdx.y into part, in channel_slice_sum's order.
-__global__ void __launch_bounds__(256) silu_bwd_bias_part(const float* z, const float* da, uint16_t* dzb,
-                                                          float* part, int P, int C, int rows) {
-    channel_slice_sum(part, P, C, rows, [&](int64_t i) {
-        const float d = z ? silu_grad(z[i], da[i]) : da[i];
-        dzb[i] = f2bf(d);
-        return d;
-    });
+template <bool ACT>
+__global__ void __launch_bounds__(256) silu_bwd_bias_part(const float* __restrict__ z, const float* __restrict__ da,
+                                                          uint16_t* __restrict__ dzb, float* __restrict__ part, int P,
+                                                          int C, int rows) {
+    channel_slice_sum(
+        part, P, C, rows, [&](int64_t i) { return float2{ACT ? z[i] : 0.f, da[i]}; },
+        [&](int64_t i, float2 t) {
+            const float d = ACT ? silu_grad(t.x, t.y) : t.y;
+            dzb[i] = f2bf(d);
+            return d;
+        });
 }
 
 __global__ void to_bf16(const float* x, uint16_t* y, int64_t n) {
@@ -1320,8 +1343,9 @@ int mpd_nn_silu_bwd_bias(const float* z, const float* da, uint16_t* dz_bf16, flo
     const int S = bias_slices(P, C);
     const int rows = (P + S - 1) / S;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(silu_bwd_bias_part, dim3((unsigned)((C + bias_cw(C) - 1) / bias_cw(C)), (unsigned)S), dim3(256),
-                       0, s, z, da, dz_bf16, work, P, C, rows);
+    const dim3 g((unsigned)((C + bias_cw(C) - 1) / bias_cw(C)), (unsigned)S);
+    if (z) hipLaunchKernelGGL(silu_bwd_bias_part<true>, g, dim3(256), 0, s, z, da, dz_bf16, work, P, C, rows);
+    else hipLaunchKernelGGL(silu_bwd_bias_part<false>, g, dim3(256), 0, s, z, da, dz_bf16, work, P, C, rows);
     hipLaunchKernelGGL(column_sum, dim3((unsigned)((C + 63) / 64)), dim3(1024), 0, s, (const float*)work, db, (int64_t)C, S);
     return finish(hipSuccess);
 }
