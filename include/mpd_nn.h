@@ -106,6 +106,12 @@ int mpd_nn_mse(const float* y, const float* t, float* dy, float* part, long long
  * 1 leaves it unchanged); w_bf16 (may be NULL) receives the bf16 copy. */
 int mpd_nn_adam(float* w, uint16_t* w_bf16, const float* g, float* m, float* v, long long n, float lr, float b1,
                 float b2, float eps, int step, float gscale, void* stream);
+/* The same step over `count` tensors (host arrays of device pointers and
+ * sizes; w_bf16 or its entries may be NULL) in as few launches as possible,
+ * bit for bit as mpd_nn_adam on each. */
+int mpd_nn_adam_multi(int count, float* const* w, uint16_t* const* w_bf16, const float* const* g, float* const* m,
+                      float* const* v, const long long* n, float lr, float b1, float b2, float eps, int step,
+                      float gscale, void* stream);
 
 /* mpd_nn_sumsq_blocks(n) partial sums of x[i]^2 in part (the caller adds
  * them in order: a deterministic squared norm). */

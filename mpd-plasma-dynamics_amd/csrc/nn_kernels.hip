@@ -1010,17 +1010,51 @@ __global__ void __launch_bounds__(256) mse(const float* y, const float* t, float
 
 // Adam (Kingma & Ba 2015) with bias correction on the gradient scaled by
 // gscale (global-norm clipping); writes the bf16 copy.
+__device__ __forceinline__ void adam_elem(float* w, uint16_t* wb, const float* g, float* m, float* v, int64_t i,
+                                          float lr, float b1, float b2, float eps, float c1, float c2, float gscale) {
+    const float gi = g[i] * gscale;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
 __global__ void adam(float* w, uint16_t* wb, const float* g, float* m, float* v, int64_t n, float lr, float b1,
                      float b2, float eps, float c1, float c2, float gscale) {
-    GRID_STRIDE(i, n) {
-        const float gi = g[i] * gscale;
-        const float mi = b1 * m[i] + (1.f - b1) * gi;
-        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-        m[i] = mi;
-        v[i] = vi;
-        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-        w[i] = wi;
-        if (wb) wb[i] = f2bf(wi);
+    GRID_STRIDE(i, n) adam_elem(w, wb, g, m, v, i, lr, b1, b2, eps, c1, c2, gscale);
+}
+
+// Adam over up to kAdamMax tensors in one launch: tensor k owns blocks
+// block0 .. block0 + ceil(n / 1024) - 1, 1024 elements each (4 per thread).
+constexpr int kAdamMax = 32;
+struct AdamSeg {
+    float* w;
+    uint16_t* wb;
+    const float* g;
+    float* m;
+    float* v;
+    int64_t n;
+    int64_t block0;
+};
+struct AdamTable {
+    AdamSeg s[kAdamMax];
+    int count;
+};
+
+__global__ void __launch_bounds__(256) adam_multi(AdamTable t, float lr, float b1, float b2, float eps, float c1,
+                                                  float c2, float gscale) {
+    const int64_t b = blockIdx.x;
+    int k = 0;
+    while (k + 1 < t.count && t.s[k + 1].block0 <= b) ++k;
+    const AdamSeg sg = t.s[k];
+    const int64_t i0 = (b - sg.block0) * 1024 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t i = i0 + 256 * j;
+        if (i < sg.n) adam_elem(sg.w, sg.wb, sg.g, sg.m, sg.v, i, lr, b1, b2, eps, c1, c2, gscale);
     }
 }
 
@@ -1469,6 +1503,30 @@ int mpd_nn_adam(float* w, uint16_t* w_bf16, const float* g, float* m, float* v, 
     const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
     hipLaunchKernelGGL(adam, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, w, w_bf16, g, m, v, (int64_t)n,
                        lr, b1, b2, eps, c1, c2, gscale);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_adam_multi(int count, float* const* w, uint16_t* const* w_bf16, const float* const* g, float* const* m,
+                      float* const* v, const long long* n, float lr, float b1, float b2, float eps, int step,
+                      float gscale, void* stream) {
+    if (count < 0 || (count > 0 && (!w || !g || !m || !v || !n)) || step < 1) return bad("adam_multi: bad argument");
+    for (int k = 0; k < count; ++k)
+        if (!w[k] || !g[k] || !m[k] || !v[k] || n[k] < 0) return bad("adam_multi: bad argument");
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    for (int k0 = 0; k0 < count; k0 += kAdamMax) {
+        AdamTable t{};
+        t.count = std::min(kAdamMax, count - k0);
+        int64_t blocks = 0;
+        for (int j = 0; j < t.count; ++j) {
+            const int k = k0 + j;
+            t.s[j] = AdamSeg{w[k], w_bf16 ? w_bf16[k] : nullptr, g[k], m[k], v[k], (int64_t)n[k], blocks};
+            blocks += (n[k] + 1023) / 1024;
+        }
+        if (blocks == 0) continue;
+        if (blocks > 0x7fffffffLL) return bad("adam_multi: too many elements");
+        hipLaunchKernelGGL(adam_multi, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, t, lr, b1, b2, eps,
+                           c1, c2, gscale);
+    }
     return finish(hipSuccess);
 }
 

@@ -75,6 +75,7 @@ def nn_lib():
         "mpd_nn_mse_blocks": [ll],
         "mpd_nn_mse": [_vp, _vp, _vp, _vp, ll, _vp],
         "mpd_nn_adam": [_vp, _vp, _vp, _vp, _vp, ll, f, f, f, f, i, f, _vp],
+        "mpd_nn_adam_multi": [i, _vp, _vp, _vp, _vp, _vp, _vp, f, f, f, f, i, f, _vp],
         "mpd_nn_sumsq_blocks": [ll],
         "mpd_nn_sumsq": [_vp, _vp, ll, _vp],
         "mpd_nn_gather_rows": [_vp, _vp, _vp, i, ll, _vp],
@@ -101,7 +102,7 @@ NN_EXPORTS = ["mpd_nn_last_error", "mpd_nn_conv3x3_fwd", "mpd_nn_conv3x3_fwd_sil
               "mpd_nn_conv3x3_bwd_weight", "mpd_nn_bias_work_floats", "mpd_nn_bias_grad", "mpd_nn_silu_fwd", "mpd_nn_silu_bwd",
               "mpd_nn_to_bf16", "mpd_nn_avgpool2_fwd", "mpd_nn_avgpool2_bwd", "mpd_nn_upsample2_fwd",
               "mpd_nn_upsample2_bwd", "mpd_nn_concat_fwd", "mpd_nn_concat_bwd", "mpd_nn_add", "mpd_nn_mse_blocks",
-              "mpd_nn_mse", "mpd_nn_adam", "mpd_nn_sumsq_blocks", "mpd_nn_sumsq", "mpd_nn_gather_rows", "mpd_nn_malloc", "mpd_nn_free",
+              "mpd_nn_mse", "mpd_nn_adam", "mpd_nn_adam_multi", "mpd_nn_sumsq_blocks", "mpd_nn_sumsq", "mpd_nn_gather_rows", "mpd_nn_malloc", "mpd_nn_free",
               "mpd_nn_memcpy_h2d", "mpd_nn_memcpy_d2h", "mpd_nn_memset", "mpd_nn_synchronize"]
 
 
@@ -447,11 +448,16 @@ class UNet:
             norm = self.grad_norm()
             if norm > clip:
                 gscale = clip / norm
-        for k in range(len(self.layers)):
-            _ck(L.mpd_nn_adam(_p(self.w[k]), _p(self.wb[k]), _p(self.gw[k]), _p(self.mw[k]), _p(self.vw[k]),
-                              self.w[k].numel(), lr, b1, b2, eps, self.step, gscale, s))
-            _ck(L.mpd_nn_adam(_p(self.b[k]), None, _p(self.gb[k]), _p(self.mb[k]), _p(self.vb[k]),
-                              self.b[k].numel(), lr, b1, b2, eps, self.step, gscale, s))
+        if getattr(self, "_adam_tab", None) is None:
+            # Every weight and bias tensor in one table (one launch per step).
+            ts = [(self.w[k], self.wb[k], self.gw[k], self.mw[k], self.vw[k]) for k in range(len(self.layers))]
+            ts += [(self.b[k], None, self.gb[k], self.mb[k], self.vb[k]) for k in range(len(self.layers))]
+            n = len(ts)
+            cols = [(_vp * n)(*[(t[j].ptr if t[j] is not None else None) for t in ts]) for j in range(5)]
+            sizes = (C.c_longlong * n)(*[t[0].numel() for t in ts])
+            self._adam_tab = (n, cols, sizes)
+        n, cols, sizes = self._adam_tab
+        _ck(L.mpd_nn_adam_multi(n, *cols, sizes, lr, b1, b2, eps, self.step, gscale, s))
         return norm
 
     def mse(self, y, t, dy):

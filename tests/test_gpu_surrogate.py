@@ -277,6 +277,51 @@ def test_fused_joins_equal_separate(nn, B, H, W, Cu, Cs):
     np.testing.assert_allclose(_host(PB2), S.pool_bwd(g.astype(np.float64)) + dy[..., Cu:], rtol=1e-6, atol=1e-7)
 
 
+def test_adam_multi_equals_separate(nn):
+    """One multi-tensor Adam launch (35 tensors: two launches of the table,
+    sizes 0 to 5000, some without a bf16 copy) equals mpd_nn_adam on each
+    tensor bit for bit, over three steps."""
+    import ctypes as C
+    L, s = nn.nn_lib(), nn._stream()
+    rng = np.random.default_rng(11)
+    sizes = [int(x) for x in rng.integers(0, 5000, size=35)]
+    sizes[3] = 0
+    sizes[7] = 1
+    sets = []
+    for _ in range(2):
+        ts = []
+        for k, n in enumerate(sizes):
+            w0 = rng.normal(size=max(n, 1)).astype(np.float32)[:n] if n else np.zeros(0, np.float32)
+            ts.append(dict(w=_dev_f32(w0) if n else _empty((1,)), wb=_empty((max(n, 1),), bf16=True) if k % 3 else None,
+                           m=_empty((max(n, 1),)), v=_empty((max(n, 1),))))
+            ts[-1]["m"].zero()
+            ts[-1]["v"].zero()
+        sets.append(ts)
+    # Same initial weights in both sets.
+    for a, b in zip(*sets):
+        if a["w"].numel() and b["w"].numel():
+            b["w"].upload(a["w"].download())
+    grads = [_dev_f32(rng.normal(size=max(n, 1)).astype(np.float32)) for n in sizes]
+    ptrs = lambda key, ts: (C.c_void_p * len(ts))(*[(t[key].ptr if t[key] is not None else None) for t in ts])
+    nn_arr = (C.c_longlong * len(sizes))(*sizes)
+    gp = (C.c_void_p * len(sizes))(*[g.ptr for g in grads])
+    for step in (1, 2, 3):
+        for k, n in enumerate(sizes):
+            t = sets[0][k]
+            nn._ck(L.mpd_nn_adam(nn._p(t["w"]), nn._p(t["wb"]) if t["wb"] is not None else None, nn._p(grads[k]),
+                                 nn._p(t["m"]), nn._p(t["v"]), n, 1e-3, 0.9, 0.999, 1e-8, step, 0.7, s))
+        nn._ck(L.mpd_nn_adam_multi(len(sizes), ptrs("w", sets[1]), ptrs("wb", sets[1]), gp, ptrs("m", sets[1]),
+                                   ptrs("v", sets[1]), nn_arr, 1e-3, 0.9, 0.999, 1e-8, step, 0.7, s))
+    for k, n in enumerate(sizes):
+        if n == 0:
+            continue
+        for key in ("w", "m", "v", "wb"):
+            a, b = sets[0][k][key], sets[1][k][key]
+            if a is None:
+                continue
+            assert np.array_equal(a.download()[:n], b.download()[:n]), (k, key)
+
+
 @pytest.mark.parametrize("base,H,W,B", [(8, 16, 8, 2), (32, 16, 8, 2), (32, 24, 40, 1)])
 def test_unet_forward_backward_match_oracle(nn, base, H, W, B):
     """The whole encoder-decoder (4 levels) on the GPU against the oracle with
