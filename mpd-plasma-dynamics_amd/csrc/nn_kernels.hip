@@ -407,8 +407,15 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
     constexpr int FM = TM / WM / 16, FN = TN / WN / 16;  // 16 x 16 blocks per wave
     constexpr int RA = (TM * 4 + 255) / 256, RB = (TN * 4 + 255) / 256;  // staging runs per thread
     constexpr bool AR = a_rows_fast<MODE>(), BR = b_rows_fast<MODE>();
-    __shared__ __attribute__((aligned(16))) uint16_t As[2][tile_elems<TM, AR>()];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][tile_elems<TN, BR>()];
+    // The staging buffers, and after the k loop the output slab of the
+    // epilogue (below), in one LDS array.
+    constexpr int A_ELEMS = tile_elems<TM, AR>(), B_ELEMS = tile_elems<TN, BR>();
+    constexpr int SP = TN + 4;        // output slab row pitch (floats)
+    constexpr int SR = TM / WM;       // output slab rows: one row of waves
+    static_assert(SR * SP * 4 <= (2 * A_ELEMS + 2 * B_ELEMS) * 2, "output slab exceeds the staging LDS");
+    __shared__ __attribute__((aligned(16))) uint16_t smem[2 * A_ELEMS + 2 * B_ELEMS];
+    uint16_t(*As)[A_ELEMS] = reinterpret_cast<uint16_t(*)[A_ELEMS]>(smem);
+    uint16_t(*Bs)[B_ELEMS] = reinterpret_cast<uint16_t(*)[B_ELEMS]>(smem + 2 * A_ELEMS);
     const int t = (int)threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave / WN, wn = wave % WN;
@@ -539,21 +546,67 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
     float* out = a.out;
     if (MODE == MODE_WGRAD) out += (int64_t)blockIdx.z * a.M * a.N;
     const int kr = (lane >> 4) * 4;
+    if (MODE != MODE_FWD) {
+        // Gradients: straight from the MFMA layout (16-float row pieces).
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
+            for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const int m = m0 + (wm * FM + i) * 16 + kr + jj;
-                const int n = n0 + (wn * FN + j) * 16 + row;
+                for (int jj = 0; jj < 4; ++jj) {
+                    const int m = m0 + (wm * FM + i) * 16 + kr + jj;
+                    const int n = n0 + (wn * FN + j) * 16 + row;
+                    if (m < a.M && n < a.N) out[(int64_t)m * a.N + n] = acc[i][j][jj];
+                }
+        return;
+    }
+    // Forward epilogue, one row of waves at a time: its accumulators go to an
+    // LDS slab of SR x TN floats (the staging buffers are free: the k loop
+    // ended on a barrier), from which each thread writes 4-float runs of
+    // output rows (whole 64- and 128-byte row pieces of z and their bf16
+    // SiLU values instead of the MFMA layout's 16-value pieces), with the
+    // bias (N is a multiple of 8, so a run is wholly inside or outside the
+    // matrix).  Measured slower for the gradients (one FP32 output).
+    float* slab = reinterpret_cast<float*>(smem);
+    constexpr int RUNS = SR * TN / 4, RPT = (RUNS + 255) / 256;
+#pragma unroll
+    for (int h = 0; h < WM; ++h) {
+        if (wm == h) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj)
+                        slab[(i * 16 + kr + jj) * SP + (wn * FN + j) * 16 + row] = acc[i][j][jj];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = t + 256 * q;
+            if (RUNS % 256 == 0 || r < RUNS) {
+                const int sr = r / (TN / 4), c4 = 4 * (r % (TN / 4));
+                const int m = m0 + h * SR + sr, n = n0 + c4;
                 if (m < a.M && n < a.N) {
-                    float v = acc[i][j][jj];
-                    if (MODE == MODE_FWD && a.bias) v += a.bias[n];
-                    out[(int64_t)m * a.N + n] = v;
-                    if (MODE == MODE_FWD && a.act) a.act[(int64_t)m * a.N + n] = silu_bf16(v);
+                    float4 v = *reinterpret_cast<const float4*>(&slab[sr * SP + c4]);
+                    if (a.bias) {
+                        const float4 b = *reinterpret_cast<const float4*>(a.bias + n);
+                        v.x += b.x;
+                        v.y += b.y;
+                        v.z += b.z;
+                        v.w += b.w;
+                    }
+                    *reinterpret_cast<float4*>(out + (int64_t)m * a.N + n) = v;
+                    if (a.act) {
+                        const uint32_t lo = (uint32_t)silu_bf16(v.x) | ((uint32_t)silu_bf16(v.y) << 16);
+                        const uint32_t hi = (uint32_t)silu_bf16(v.z) | ((uint32_t)silu_bf16(v.w) << 16);
+                        *reinterpret_cast<uint2*>(a.act + (int64_t)m * a.N + n) = uint2{lo, hi};
+                    }
                 }
             }
+        }
+        __syncthreads();
+    }
 }
 
 // The weight gradient from LDS-resident row blocks, for channel counts that
