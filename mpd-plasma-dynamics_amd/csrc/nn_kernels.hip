@@ -1180,13 +1180,16 @@ Tile pick_tile(int M, int N) {
 // most CUs idle, one 128 x 128 tile per 64 k values of latency each.
 Tile pick_tile_fill(int M, int N) {
     Tile t = pick_tile(M, N);
+    // Multiples of 96 (the decoders' concatenated inputs, 96 and 192
+    // channels) in 96-wide tiles: no idle quarter of a 128-wide one.
+    if (N > 64 && N % 96 == 0 && N % 128 != 0) t.tn = 96;
     auto count = [&](const Tile& u) { return (long long)((M + u.tm - 1) / u.tm) * ((N + u.tn - 1) / u.tn); };
     // Rows halve while at least 32 and at least the tile's width (the
     // instantiated shapes below have 64 rows or more when 16 wide).
     auto rows_ok = [&](const Tile& u) { return u.tm > 32 && (u.tn >= 32 || u.tm > 64); };
     while (count(t) < 1024) {
         if (rows_ok(t) && t.tm >= t.tn) t.tm /= 2;
-        else if (t.tn > 64) t.tn /= 2;
+        else if (t.tn > 64) t.tn = 64;
         else if (rows_ok(t)) t.tm /= 2;
         else break;
     }
@@ -1271,14 +1274,17 @@ void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
 #define MPD_GEMM2(TM, TN, WM, WN) \
     if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm2<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return; }
     MPD_GEMM2(128, 128, 2, 2)
+    MPD_GEMM2(128, 96, 2, 2)
     MPD_GEMM2(128, 64, 2, 2)
     MPD_GEMM2(128, 32, 4, 1)
     MPD_GEMM2(128, 16, 4, 1)
     MPD_GEMM2(64, 128, 2, 2)
+    MPD_GEMM2(64, 96, 2, 2)
     MPD_GEMM2(64, 64, 2, 2)
     MPD_GEMM2(64, 32, 4, 1)
     MPD_GEMM2(64, 16, 4, 1)
     MPD_GEMM2(32, 128, 2, 2)
+    MPD_GEMM2(32, 96, 2, 2)
     MPD_GEMM2(32, 64, 2, 2)
     MPD_GEMM2(32, 32, 2, 2)
     MPD_GEMM2(16, 128, 1, 4)
