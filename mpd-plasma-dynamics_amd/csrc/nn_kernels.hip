@@ -419,7 +419,18 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
     const int t = (int)threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave / WN, wn = wave % WN;
-    const int m0 = (int)blockIdx.x * TM, n0 = (int)blockIdx.y * TN;
+    // Pixel tiles (forward, data gradient): workgroups go to the 8 XCDs
+    // round-robin (by linear index; gridDim.x tiles per row of the grid), so
+    // XCD x is given the x-th eighth of the tiles in order -- the
+    // neighbouring tiles whose tap gathers share image rows then meet in the
+    // same L2.  Eighths of q or q + 1 tiles (the first r = nx mod 8 longer): a
+    // bijection of 0 .. nx - 1.
+    int bx = (int)blockIdx.x;
+    if (MODE != MODE_WGRAD) {
+        const int nx = (int)gridDim.x, q = nx >> 3, r = nx & 7, x = bx & 7;
+        bx = x * q + min(x, r) + (bx >> 3);
+    }
+    const int m0 = bx * TM, n0 = (int)blockIdx.y * TN;
     int k_begin = 0, k_end = a.K;
     if (MODE == MODE_WGRAD) {
         k_begin = (int)blockIdx.z * a.kchunk;
