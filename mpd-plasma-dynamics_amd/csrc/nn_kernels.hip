@@ -661,8 +661,17 @@ __global__ void __launch_bounds__(256, 2) wgrad_lds(WgArgs a) {
     const int KQ = (RC * W + 31) / 32 * 32;
     uint16_t* xs = lds;
     uint16_t* ds = lds + (RC + 2) * W2 * WG_XP;
-    const int co0 = (int)blockIdx.x * CO_T, ci0 = (int)blockIdx.y * WG_CI;
-    const int sl = (int)blockIdx.z, b = sl / a.spi;
+    // Workgroups go to the 8 XCDs round-robin by linear index; the tiles are
+    // handed out so that XCD x takes the x-th eighth of the (slice, tile)
+    // list in order: the tiles of one slice, which stage the same dz and x
+    // rows, then meet in one L2.
+    const int nco = (int)gridDim.x, nci = (int)gridDim.y;
+    const int n = nco * nci * (int)gridDim.z;
+    const int lin = (int)blockIdx.x + nco * ((int)blockIdx.y + nci * (int)blockIdx.z);
+    const int lq = n >> 3, lr = n & 7, lx = lin & 7;
+    const int lg = lx * lq + min(lx, lr) + (lin >> 3);
+    const int co0 = (lg % nco) * CO_T, ci0 = ((lg / nco) % nci) * WG_CI;
+    const int sl = lg / (nco * nci), b = sl / a.spi;
     const int y0 = (sl - b * a.spi) * a.rps, y1 = min(a.H, y0 + a.rps);
     const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
     const int s = lane & 15, g = lane >> 4;
