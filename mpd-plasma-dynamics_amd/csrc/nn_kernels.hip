@@ -17,7 +17,12 @@
 // (channel counts multiples of 8): tiles up to 128 x 128 on
 // v_mfma_f32_16x16x32_bf16 from 16-byte runs.  wgrad_lds (the weight
 // gradient, channel counts multiples of 16): row blocks staged in LDS and
-// read with gfx950's LDS transpose reads.
+// read with gfx950's LDS transpose reads.  Fused passes for the training
+// step: the forward GEMM writes SiLU's output beside z, the SiLU backward
+// writes dz with the bias gradient's slices, the decoder's upsample +
+// concatenation (and its backward) and the encoder's pooling backward +
+// skip gradient are one pass each, Adam runs over every tensor at once;
+// each equals its separate passes bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
