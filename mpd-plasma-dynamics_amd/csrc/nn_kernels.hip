@@ -625,6 +625,209 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
     }
 }
 
+// The forward and data-gradient GEMMs with the three horizontal taps taken
+// from one staged slab (the gathered operand's channel count a multiple of
+// 32, N tiles of 32 or 64): stage s = (kernel row ky, 32-channel chunk cc)
+// stages the TM + 2 consecutive pixels m0 - 1 + j + sdy W (j < TM + 2; sdy
+// the row shift of ky) at the chunk's 32 channels, and the B tiles of the
+// row's three taps; tap kx then reads the A fragment of output pixel m at
+// staged pixel m - m0 + 1 + sdx (sdx its column shift).  A staged pixel is
+// the tap's source pixel exactly when that source lies in the image; a lane
+// whose source does not (the image's edge columns and rows, pixels past M)
+// takes a zero fragment instead.  The gathered operand is read about 3 times
+// per output tile instead of 9.  Forward: sdy = ky - 1, sdx = kx - 1, B(n,
+// k) = w[n][tap][ci]; data gradient: sdy = 1 - ky, sdx = 1 - kx, B(n = ci,
+// k = (tap, co)) = w[co][tap][ci] staged in source order.  Same pipeline as
+// conv_gemm2 (two LDS stages, the next stage's runs loaded into registers
+// while the current one is computed), same epilogues.
+template <int MODE, int TM, int TN, int WM, int WN>
+__global__ void __launch_bounds__(256) conv_gemm3(ConvArgs a) {
+    static_assert(MODE == MODE_FWD || MODE == MODE_DGRAD, "pixel-row GEMMs only");
+    static_assert(WM * WN == 4 && TM % (16 * WM) == 0 && TN % (16 * WN) == 0 && TN <= 64, "tile shape");
+    constexpr int FM = TM / WM / 16, FN = TN / WN / 16;
+    constexpr bool BR = MODE == MODE_DGRAD;
+    constexpr int AROWS = TM + 2, A_ELEMS = AROWS * LDK2, B_ELEMS = tile_elems<TN, BR>();
+    constexpr int STAGE = A_ELEMS + 3 * B_ELEMS;
+    constexpr int RA = (AROWS * 4 + 255) / 256, RB = (3 * TN * 4 + 255) / 256;
+    constexpr int SP = TN + 4, SR = TM / WM;
+    static_assert(SR * SP * 4 <= 2 * STAGE * 2, "output slab exceeds the staging LDS");
+    __shared__ __attribute__((aligned(16))) uint16_t smem[2 * STAGE];
+    const int t = (int)threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int s16 = lane & 15, g = lane >> 4;
+    int bx = (int)blockIdx.x;
+    {
+        const int nx = (int)gridDim.x, q = nx >> 3, r = nx & 7, x = bx & 7;
+        bx = x * q + min(x, r) + (bx >> 3);
+    }
+    const int m0 = bx * TM, n0 = (int)blockIdx.y * TN;
+    const int Cg = MODE == MODE_FWD ? a.Cin : a.Cout;  // channels of the gathered operand
+    const uint16_t* src = MODE == MODE_FWD ? a.x : a.dz;
+    const int ncc = Cg >> 5, nstage = 3 * ncc;
+    // This lane's output pixels (one per M fragment): image row and column,
+    // or y = -(1 << 20) past M (every tap then invalid).
+    int py[FM], px[FM];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int m = m0 + (wm * FM + i) * 16 + s16;
+        if (m < a.M) {
+            const int r = m % (a.H * a.W);
+            py[i] = r / a.W;
+            px[i] = r - py[i] * a.W;
+        } else {
+            py[i] = -(1 << 20);
+            px[i] = 0;
+        }
+    }
+    f32x4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    uint4 ra[RA], rb[RB];
+    auto fetch = [&](int st) {
+        const int ky = st / ncc, cc = st - ky * ncc;
+        const int sdy = MODE == MODE_FWD ? ky - 1 : 1 - ky;
+        const int64_t f0 = (int64_t)m0 - 1 + (int64_t)sdy * a.W;
+#pragma unroll
+        for (int q = 0; q < RA; ++q) {
+            const int r = t + 256 * q;
+            const int64_t f = f0 + (r >> 2);
+            ra[q] = (r < AROWS * 4 && f >= 0 && f < a.M)
+                        ? *reinterpret_cast<const uint4*>(src + f * Cg + 32 * cc + 8 * (r & 3))
+                        : zero_run();
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int r = t + 256 * q;
+            uint4 v = zero_run();
+            if (r < 3 * TN * 4) {
+                const int kx = r / (TN * 4), rr = r - kx * (TN * 4), tap = 3 * ky + kx;
+                if (MODE == MODE_FWD) {
+                    const int n = n0 + (rr >> 2);
+                    if (n < a.N) v = *reinterpret_cast<const uint4*>(a.w + (int64_t)n * a.K + tap * a.Cin + 32 * cc +
+                                                                     8 * (rr & 3));
+                } else {
+                    const int co = 32 * cc + rr / (TN / 8), n = n0 + 8 * (rr % (TN / 8));
+                    if (n < a.N) v = *reinterpret_cast<const uint4*>(a.w + ((int64_t)co * 9 + tap) * a.Cin + n);
+                }
+            }
+            rb[q] = v;
+        }
+    };
+    auto put = [&](int buf) {
+        uint16_t* A = smem + buf * STAGE;
+        uint16_t* Bb = A + A_ELEMS;
+#pragma unroll
+        for (int q = 0; q < RA; ++q) {
+            const int r = t + 256 * q;
+            if (r < AROWS * 4) *reinterpret_cast<uint4*>(&A[(r >> 2) * LDK2 + 8 * (r & 3)]) = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int r = t + 256 * q;
+            if (r < 3 * TN * 4) {
+                const int kx = r / (TN * 4), rr = r - kx * (TN * 4);
+                stage_store2<TN, BR>(rr, Bb + kx * B_ELEMS, rb[q]);
+            }
+        }
+    };
+    auto mma = [&](int st, int buf) {
+        const int ky = st / ncc;
+        const int sdy = MODE == MODE_FWD ? ky - 1 : 1 - ky;
+        const uint16_t* A = smem + buf * STAGE;
+        const uint16_t* Bb = A + A_ELEMS;
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) {
+            const int sdx = MODE == MODE_FWD ? kx - 1 : 1 - kx;
+            bf16x8v fa[FM], fb[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                const int y = py[i] + sdy, x = px[i] + sdx;
+                const bool ok = y >= 0 && y < a.H && x >= 0 && x < a.W;
+                const int row = (wm * FM + i) * 16 + s16 + 1 + sdx;
+                const uint4 v = *reinterpret_cast<const uint4*>(&A[row * LDK2 + 8 * g]);
+                fa[i] = __builtin_bit_cast(bf16x8v, ok ? v : zero_run());
+            }
+#pragma unroll
+            for (int j = 0; j < FN; ++j) fb[j] = frag2<TN, BR>(Bb + kx * B_ELEMS, (wn * FN + j) * 16, lane);
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+    };
+    fetch(0);
+    put(0);
+    __syncthreads();
+    int buf = 0;
+    for (int st = 0; st < nstage; ++st) {
+        const bool more = st + 1 < nstage;
+        if (more) fetch(st + 1);
+        mma(st, buf);
+        if (more) put(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    const int kr = g * 4;
+    if (MODE == MODE_DGRAD) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const int m = m0 + (wm * FM + i) * 16 + kr + jj;
+                    const int n = n0 + (wn * FN + j) * 16 + s16;
+                    if (m < a.M && n < a.N) a.out[(int64_t)m * a.N + n] = acc[i][j][jj];
+                }
+        return;
+    }
+    // Forward epilogue through an LDS slab, as in conv_gemm2.
+    float* slab = reinterpret_cast<float*>(smem);
+    constexpr int RUNS = SR * TN / 4, RPT = (RUNS + 255) / 256;
+#pragma unroll
+    for (int h = 0; h < WM; ++h) {
+        if (wm == h) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj)
+                        slab[(i * 16 + kr + jj) * SP + (wn * FN + j) * 16 + s16] = acc[i][j][jj];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = t + 256 * q;
+            if (RUNS % 256 == 0 || r < RUNS) {
+                const int sr = r / (TN / 4), c4 = 4 * (r % (TN / 4));
+                const int m = m0 + h * SR + sr, n = n0 + c4;
+                if (m < a.M && n < a.N) {
+                    float4 v = *reinterpret_cast<const float4*>(&slab[sr * SP + c4]);
+                    if (a.bias) {
+                        const float4 b = *reinterpret_cast<const float4*>(a.bias + n);
+                        v.x += b.x;
+                        v.y += b.y;
+                        v.z += b.z;
+                        v.w += b.w;
+                    }
+                    *reinterpret_cast<float4*>(a.out + (int64_t)m * a.N + n) = v;
+                    if (a.act) {
+                        const uint32_t lo = (uint32_t)silu_bf16(v.x) | ((uint32_t)silu_bf16(v.y) << 16);
+                        const uint32_t hi = (uint32_t)silu_bf16(v.z) | ((uint32_t)silu_bf16(v.w) << 16);
+                        *reinterpret_cast<uint2*>(a.act + (int64_t)m * a.N + n) = uint2{lo, hi};
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // The weight gradient from LDS-resident row blocks, for channel counts that
 // are multiples of 16: dw[co][tap][ci] = sum over pixels p of dz[p][co]
 // x[p + shift(tap)][ci].  The pixel sum is the GEMM's k, against the NHWC
@@ -1296,6 +1499,21 @@ template <int MODE>
 void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
     const Tile t = MODE == MODE_WGRAD ? pick_tile(a.M, a.N) : pick_tile_fill(a.M, a.N);
     const dim3 g((unsigned)((a.M + t.tm - 1) / t.tm), (unsigned)((a.N + t.tn - 1) / t.tn), (unsigned)splits);
+    if constexpr (MODE != MODE_WGRAD) {
+        // Gathered channels in 32-channel chunks: the three-tap slab kernel
+        // for the tiles where it measured faster (32-wide tiles and 32-pixel
+        // tiles; the 64 x 64 and 128 x 64 slab tiles hold three taps of B in
+        // 41-52 KB of LDS, and the lower occupancy made them 4-5 % slower).
+        if ((MODE == MODE_FWD ? a.Cin : a.Cout) % 32 == 0) {
+#define MPD_GEMM3(TM, TN, WM, WN) \
+    if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm3<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return; }
+            MPD_GEMM3(128, 32, 4, 1)
+            MPD_GEMM3(64, 32, 4, 1)
+            MPD_GEMM3(32, 64, 2, 2)
+            MPD_GEMM3(32, 32, 2, 2)
+#undef MPD_GEMM3
+        }
+    }
 #define MPD_GEMM2(TM, TN, WM, WN) \
     if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm2<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return; }
     MPD_GEMM2(128, 128, 2, 2)
