@@ -58,11 +58,15 @@ def _empty(shape, bf16=False):
 # (wgrad_lds): shapes with several chunks and slices per image, rows of 21
 # and of 300 pixels (one row per chunk), and 400-pixel rows, too wide for
 # its LDS tiles (the GEMM path then); and rows of 21, 84 and 168 pixels,
-# the network's levels.
+# the network's levels.  Gathered channels in 32-channel chunks take the
+# three-tap slab kernel on its tiles.
 SHAPES = [(2, 8, 5, 3, 5), (3, 12, 10, 8, 32), (1, 16, 8, 96, 32), (2, 6, 5, 40, 70), (1, 5, 3, 256, 130),
           (2, 9, 7, 16, 8), (1, 20, 24, 64, 128), (4, 3, 2, 32, 256), (1, 4, 4, 128, 16), (2, 10, 6, 384, 128),
           (2, 40, 21, 32, 32), (1, 9, 300, 16, 16), (1, 4, 400, 16, 32), (1, 5, 168, 32, 32),
-          (1, 6, 84, 64, 64), (2, 13, 21, 64, 48)]
+          (1, 6, 84, 64, 64), (2, 13, 21, 64, 48),
+          # one-pixel-wide and one-row images (every shifted tap of one axis
+          # leaves the image), and 96 channels (the slab kernel's three chunks)
+          (8, 40, 1, 32, 32), (3, 1, 9, 64, 32), (1, 6, 10, 96, 32)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
