@@ -216,9 +216,11 @@ int mpd_batch_instance(mpd_batch* b, int i, mpd_solver** out);
  * same bits either way. */
 int mpd_batch_set_option(mpd_batch* b, const char* key, double value);
 
-/* Autotuner: times the batched stencil launch on the current states (reps
+/* Autotuner: times the batched stencil launch on the current states of the
+ * instances whose loops have not ended (all of them if none is left; reps
  * launches per trial) and picks the tile height of each distinct grid size
- * and the stencil grid by coordinate search; writes a text report. */
+ * and the stencil grid by coordinate search; writes a text report.  Called
+ * again when few instances remain, it tunes the batch's tail. */
 int mpd_batch_autotune(mpd_batch* b, int reps, char* report, int report_len);
 int mpd_batch_tile_rows(const mpd_batch* b, int i, int* tile_rows);
 

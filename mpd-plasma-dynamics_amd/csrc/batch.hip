@@ -468,11 +468,10 @@ struct mpd_batch {
     // current tile heights and grid (explicit dt: no loop state is read or
     // finished; the launch writes only each instance's next-state buffer and
     // row buffer, which the next step rewrites before reading).
-    double TimeStencil(int reps) {
-        std::vector<char> all(n(), 1);
+    double TimeStencil(int reps, const std::vector<char>& live) {
         std::vector<mpd::BatchBlock> w;
         int a = 0, b = 0, c = 0;
-        BuildWork(all, w, a, b, c);
+        BuildWork(live, w, a, b, c);
         UploadWork(w);
         for (int i = 0; i < n(); ++i) h_icur.p[i] = inst[i]->cur;
         HIPCHK(hipMemcpyAsync(d_icur, h_icur.p, sizeof(int) * n(), hipMemcpyHostToDevice, stream));
@@ -495,19 +494,30 @@ struct mpd_batch {
     }
 
     // Coordinate search: the tile height of each distinct grid size (largest
-    // first, the others held), then the stencil grid.  The work lists of the
-    // next run are rebuilt with the result.
+    // first, the others held), then the stencil grid, timed over the
+    // instances whose loops have not ended (all of them if none is left):
+    // tuned again when few instances remain, the batch's tail gets the
+    // heights that suit it.  The work lists of the next run are rebuilt with
+    // the result.
     void Autotune(int reps) {
         reps = std::max(1, reps);
+        std::vector<char> live(n(), 0);
+        bool any = false;
+        for (int i = 0; i < n(); ++i) {
+            live[i] = inst[i]->LoopSeed(0).done ? 0 : 1;
+            any = any || live[i];
+        }
+        if (!any) live.assign(n(), 1);
         // Instances of one grid size share a tile height.
         std::map<std::tuple<long long, int, int>, std::vector<int>, std::greater<>> classes;
         for (int i = 0; i < n(); ++i) {
+            if (!live[i]) continue;
             const mpd_solver& s = *inst[i];
             classes[{(long long)s.grid.L * s.grid.Mn, s.grid.L, s.grid.Mn}].push_back(i);
         }
         std::string rep;
         char line[256];
-        double best = TimeStencil(reps);
+        double best = TimeStencil(reps, live);
         std::snprintf(line, sizeof line, "start: %.4f ms per batched stencil launch\n", best);
         rep += line;
         for (auto& kv : classes) {
@@ -518,7 +528,7 @@ struct mpd_batch {
                 if (cand > s0.grid.L) continue;
                 if (cand == bestH) continue;
                 for (int i : ids) H[i] = cand;
-                const double t = TimeStencil(reps);
+                const double t = TimeStencil(reps, live);
                 if (t < best) {
                     best = t;
                     bestH = cand;
@@ -533,7 +543,7 @@ struct mpd_batch {
         for (int g : kGridCandidates) {
             if (g == bestG) continue;
             grid = g;
-            const double t = TimeStencil(reps);
+            const double t = TimeStencil(reps, live);
             if (t < best) {
                 best = t;
                 bestG = g;

@@ -151,10 +151,13 @@ def converge_uneven(mpd_amd, sizes, n=64, seed=2, threshold=1e-8, plain_sample=1
     cells = sum(int(z) * (max(2, int(round(0.4 * z))) + 1) for z in nz)
     finish = np.full(n, np.nan)
     steps = np.zeros(n, int)
+    sizes_of = np.array([int(z) * (max(2, int(round(0.4 * z))) + 1) for z in nz])
     with mpd_amd.Batch(cfgs) as b:
         b.autotune(5)
         t0 = time.perf_counter()
         last = 0.0
+        tuned_cells = cells
+        retunes = 0
         while True:
             res = b.run(chunk)
             el = time.perf_counter() - t0
@@ -163,6 +166,14 @@ def converge_uneven(mpd_amd, sizes, n=64, seed=2, threshold=1e-8, plain_sample=1
                 if r["done"] and np.isnan(finish[i]):
                     finish[i] = el
             nd = sum(r["done"] for r in res)
+            # When the running cases hold under a quarter of the cells the
+            # tile heights were tuned for, tune them again (inside the timed
+            # run): the tail of a few large cases wants shorter tiles.
+            live_cells = int(sum(sizes_of[i] for i, r in enumerate(res) if not r["done"]))
+            if 0 < live_cells < tuned_cells // 4:
+                b.autotune(3)
+                tuned_cells = live_cells
+                retunes += 1
             if el - last >= 10.0 or nd == n:
                 print(f"[sweep] batched: {el:.1f} s, {nd}/{n} done, max steps {steps.max()}", flush=True)
                 last = el
@@ -192,4 +203,5 @@ def converge_uneven(mpd_amd, sizes, n=64, seed=2, threshold=1e-8, plain_sample=1
             "one_at_a_time_converged": plain_conv,
             "one_at_a_time_mean_s_by_size": mean, "speedup": plain_s / batch_s,
             "steps_p50": float(np.median(steps)), "steps_max": int(steps.max()),
-            "finish_p50_s": float(np.nanmedian(finish)), "finish_p99_s": float(np.nanpercentile(finish, 99))}
+            "finish_p50_s": float(np.nanmedian(finish)), "finish_p99_s": float(np.nanpercentile(finish, 99)),
+            "retunes": retunes}
