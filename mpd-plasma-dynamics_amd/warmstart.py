@@ -51,6 +51,8 @@ def solve_batch(configs, warm_u0=None, chunk=0, label="batch"):
                 b.warm_start(i, u)
     b.autotune(5)
     n = len(configs)
+    cells = [(lm[0]) * (lm[1] + 1) for lm in (mpd_amd.config_check(c) for c in configs)]
+    tuned_cells = sum(cells)
     steps = np.zeros(n, int)
     finish = np.full(n, np.nan)
     t0 = time.perf_counter()
@@ -63,6 +65,12 @@ def solve_batch(configs, warm_u0=None, chunk=0, label="batch"):
             if r["done"] and np.isnan(finish[i]):
                 finish[i] = el
         ndone = sum(r["done"] for r in res)
+        # The running cases' tile heights tuned again (inside the timed run)
+        # when they hold under a quarter of the cells of the last tuning.
+        live = sum(c for c, r in zip(cells, res) if not r["done"])
+        if chunk and 0 < live < tuned_cells // 4:
+            b.autotune(3)
+            tuned_cells = live
         if chunk and (el - last >= 10.0 or ndone == n):
             progress(f"{label}: {el:.1f}s, {ndone}/{n} done, max steps {steps.max()}")
             last = el
