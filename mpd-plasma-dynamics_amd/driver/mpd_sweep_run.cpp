@@ -72,7 +72,38 @@ int main(int argc, char** argv) {
         }
         std::vector<mpd_run_result> res(n);
         const auto t0 = std::chrono::steady_clock::now();
-        Check(mpd_batch_run(b, max_steps, res.data()));
+        if (max_steps > 0 || !autotune) {
+            Check(mpd_batch_run(b, max_steps, res.data()));
+        } else {
+            // To the loops' own ends in slices of kSlice steps (the same bits
+            // as one run): when the cases still running hold under a quarter
+            // of the cells the tile heights were tuned for, tune them again
+            // -- the tail of a few large cases wants shorter tiles.
+            constexpr int kSlice = 20000;
+            std::vector<long long> cells(n);
+            long long tuned = 0;
+            for (int i = 0; i < n; ++i) {
+                mpd_solver* s = nullptr;
+                Check(mpd_batch_instance(b, i, &s));
+                int L = 0, M = 0, rows = 0, cols = 0;
+                Check(mpd_dims(s, &L, &M, &rows, &cols));
+                cells[i] = (long long)L * (M + 1);
+                tuned += cells[i];
+            }
+            for (;;) {
+                Check(mpd_batch_run(b, kSlice, res.data()));
+                long long live = 0;
+                for (int i = 0; i < n; ++i)
+                    if (!res[i].done) live += cells[i];
+                if (live == 0) break;
+                if (live < tuned / 4) {
+                    char report[4096];
+                    Check(mpd_batch_autotune(b, 3, report, (int)sizeof report));
+                    std::printf("[sweep] autotuner (%lld cells still running):\n%s", live, report);
+                    tuned = live;
+                }
+            }
+        }
         const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::printf("[sweep] all cases done in %.3f s\n", wall);
         std::printf("%-5s %-28s %-11s %-10s %-14s %-14s %-14s %-9s %-14s %-14s\n", "case", "config", "grid", "steps",
@@ -89,7 +120,7 @@ int main(int argc, char** argv) {
             std::snprintf(grid, sizeof grid, "%dx%d", L, M);
             const std::string name = std::filesystem::path(paths[i]).filename().string();
             std::printf("%-5d %-28s %-11s %-10d %-14.6e %-14.6e %-14.6e %-9s %-14.6e %-14.6e\n", i, name.c_str(), grid,
-                        res[i].steps, res[i].t, res[i].dt, res[i].change, res[i].converged ? "yes" : "no", mdot,
+                        res[i].total_steps, res[i].t, res[i].dt, res[i].change, res[i].converged ? "yes" : "no", mdot,
                         thrust);
             if (!out_dir.empty()) {
                 char file[64];
