@@ -48,6 +48,16 @@ int mpd_nn_conv3x3_fwd_silu(const uint16_t* x, const uint16_t* w, const float* b
 /* dx = dL/dx [B][H][W][Cin] FP32 from dz = dL/dz [B][H][W][Cout] bf16. */
 int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, int B, int H, int W, int Cin, int Cout,
                             void* stream);
+/* The data gradient with the layer below's SiLU backward fused (channel
+ * counts multiples of 8, Cin not an odd multiple of 96): for dx as above and
+ * z_in that layer's pre-activation [B][H][W][Cin], writes d = dx silu'(z_in)
+ * as bf16 (dz_in_bf16; bit for bit mpd_nn_silu_bwd_bias's) and db_in[c] =
+ * the sum of d over the pixels (FP32, per pixel tile then over the tiles in
+ * a fixed order: deterministic, though not mpd_nn_silu_bwd_bias's order); no
+ * FP32 dx is written.  `work` holds mpd_nn_bwd_data_silu_work_floats() floats. */
+long long mpd_nn_bwd_data_silu_work_floats(int B, int H, int W, int Cin, int Cout);
+int mpd_nn_conv3x3_bwd_data_silu(const uint16_t* dz, const uint16_t* w, const float* z_in, uint16_t* dz_in_bf16,
+                                 float* db_in, float* work, int B, int H, int W, int Cin, int Cout, void* stream);
 /* dw = dL/dw [Cout][3][3][Cin] FP32; `work` holds mpd_nn_wgrad_work_floats()
  * floats of split partial sums (reduced in a fixed order). */
 long long mpd_nn_wgrad_work_floats(int B, int H, int W, int Cin, int Cout);

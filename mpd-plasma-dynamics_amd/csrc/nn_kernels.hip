@@ -67,6 +67,12 @@ struct ConvArgs {
     const float* bias;   // [Cout] (forward)
     float* out;          // forward: z [P][Cout]; data grad: dx [P][Cin]; weight grad: partials [S][Cout][9 Cin]
     uint16_t* act;       // forward: bf16(silu(z)) [P][Cout] as well, or null
+    // Data gradient with the SiLU backward of the layer below fused (zact
+    // non-null): dzo = bf16(out * silu'(zact)) [P][Cin] instead of out, and
+    // per pixel tile column sums of those values into bpart [tiles][Cin].
+    const float* zact;
+    uint16_t* dzo;
+    float* bpart;
     int B, H, W, Cin, Cout;
     int M, N, K;
     int kchunk;  // weight grad: pixels per split
@@ -360,6 +366,58 @@ __device__ __forceinline__ bf16x8v frag2(const uint16_t* S, int rb, int lane) {
     return __builtin_bit_cast(bf16x8v, *reinterpret_cast<const uint4*>(&S[(rb + s) * LDK2 + 8 * g]));
 }
 
+// The data gradient's epilogue with the SiLU backward of the layer below,
+// one row of waves at a time through an LDS slab (as the forward's): each
+// thread takes 4-float runs of output rows (its column group fixed, 256 being
+// a multiple of the TN / 4 groups), forms d = dx silu'(z) from whole 16-byte
+// pieces of z, writes bf16 d, and adds d to its own 4 column sums; the sums
+// of the threads of one column group are then added in thread order, and the
+// tile's column sums go to bpart[tile_m] -- a deterministic order.
+template <int WM, int TM, int TN>
+__device__ __forceinline__ void dgrad_silu_rows(const ConvArgs& a, float* slab, int m0, int n0, int h, float4& cs) {
+    constexpr int SP = TN + 4, SR = TM / WM, CG = TN / 4;
+    constexpr int RUNS = SR * CG, RPT = (RUNS + 255) / 256;
+    const int t = (int)threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = t + 256 * q;
+        if (RUNS % 256 == 0 || r < RUNS) {
+            const int sr = r / CG, c4 = 4 * (r % CG);
+            const int m = m0 + h * SR + sr, n = n0 + c4;
+            if (m < a.M && n < a.N) {
+                const int64_t e = (int64_t)m * a.N + n;
+                const float4 v = *reinterpret_cast<const float4*>(&slab[sr * SP + c4]);
+                const float4 z = *reinterpret_cast<const float4*>(a.zact + e);
+                const float d0 = silu_grad(z.x, v.x), d1 = silu_grad(z.y, v.y);
+                const float d2 = silu_grad(z.z, v.z), d3 = silu_grad(z.w, v.w);
+                const uint32_t lo = (uint32_t)f2bf(d0) | ((uint32_t)f2bf(d1) << 16);
+                const uint32_t hi = (uint32_t)f2bf(d2) | ((uint32_t)f2bf(d3) << 16);
+                *reinterpret_cast<uint2*>(a.dzo + e) = uint2{lo, hi};
+                cs.x += d0;
+                cs.y += d1;
+                cs.z += d2;
+                cs.w += d3;
+            }
+        }
+    }
+}
+
+template <int TN>
+__device__ __forceinline__ void dgrad_silu_sums(const ConvArgs& a, float* red, int n0, int tile_m, const float4& cs) {
+    constexpr int CG = TN / 4;
+    static_assert(256 % CG == 0, "a thread's column group must stay fixed");
+    const int t = (int)threadIdx.x;
+    *reinterpret_cast<float4*>(&red[4 * t]) = cs;
+    __syncthreads();
+    for (int c = t; c < TN; c += 256) {
+        const int n = n0 + c;
+        if (n >= a.N) continue;
+        float v = 0.f;
+        for (int k = 0; k < 256 / CG; ++k) v += red[4 * ((c >> 2) + CG * k) + (c & 3)];
+        a.bpart[(int64_t)tile_m * a.N + n] = v;
+    }
+}
+
 // Index k = tap * C + c of a run along a (tap, channel) axis, advanced tile
 // by tile without division.
 struct TapChan {
@@ -562,6 +620,29 @@ __global__ void __launch_bounds__(256) conv_gemm2(ConvArgs a) {
     float* out = a.out;
     if (MODE == MODE_WGRAD) out += (int64_t)blockIdx.z * a.M * a.N;
     const int kr = (lane >> 4) * 4;
+    if constexpr (MODE == MODE_DGRAD && 256 % (TN / 4) == 0) {
+        if (a.zact) {
+            float* slab = reinterpret_cast<float*>(smem);
+            float4 cs = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int h = 0; h < WM; ++h) {
+                if (wm == h) {
+#pragma unroll
+                    for (int i = 0; i < FM; ++i)
+#pragma unroll
+                        for (int j = 0; j < FN; ++j)
+#pragma unroll
+                            for (int jj = 0; jj < 4; ++jj)
+                                slab[(i * 16 + kr + jj) * SP + (wn * FN + j) * 16 + row] = acc[i][j][jj];
+                }
+                __syncthreads();
+                dgrad_silu_rows<WM, TM, TN>(a, slab, m0, n0, h, cs);
+                __syncthreads();
+            }
+            dgrad_silu_sums<TN>(a, slab, n0, bx, cs);
+            return;
+        }
+    }
     if (MODE != MODE_FWD) {
         // Gradients: straight from the MFMA layout (16-float row pieces).
 #pragma unroll
@@ -772,6 +853,29 @@ __global__ void __launch_bounds__(256) conv_gemm3(ConvArgs a) {
         buf ^= 1;
     }
     const int kr = g * 4;
+    if constexpr (MODE == MODE_DGRAD) {
+        if (a.zact) {
+            float* slab = reinterpret_cast<float*>(smem);
+            float4 cs = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int h = 0; h < WM; ++h) {
+                if (wm == h) {
+#pragma unroll
+                    for (int i = 0; i < FM; ++i)
+#pragma unroll
+                        for (int j = 0; j < FN; ++j)
+#pragma unroll
+                            for (int jj = 0; jj < 4; ++jj)
+                                slab[(i * 16 + kr + jj) * SP + (wn * FN + j) * 16 + s16] = acc[i][j][jj];
+                }
+                __syncthreads();
+                dgrad_silu_rows<WM, TM, TN>(a, slab, m0, n0, h, cs);
+                __syncthreads();
+            }
+            dgrad_silu_sums<TN>(a, slab, n0, bx, cs);
+            return;
+        }
+    }
     if (MODE == MODE_DGRAD) {
 #pragma unroll
         for (int i = 0; i < FM; ++i)
@@ -1496,7 +1600,7 @@ bool wgrad_lds_plan(int B, int H, int W, int Cin, int Cout, WgPlan& p) {
 // Launch of conv_gemm2 for C = a.M x a.N with the tile pick_tile (weight
 // gradient: its splits are sized to that tile) or pick_tile_fill chooses.
 template <int MODE>
-void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
+int launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
     const Tile t = MODE == MODE_WGRAD ? pick_tile(a.M, a.N) : pick_tile_fill(a.M, a.N);
     const dim3 g((unsigned)((a.M + t.tm - 1) / t.tm), (unsigned)((a.N + t.tn - 1) / t.tn), (unsigned)splits);
     if constexpr (MODE != MODE_WGRAD) {
@@ -1506,7 +1610,7 @@ void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
         // 41-52 KB of LDS, and the lower occupancy made them 4-5 % slower).
         if ((MODE == MODE_FWD ? a.Cin : a.Cout) % 32 == 0) {
 #define MPD_GEMM3(TM, TN, WM, WN) \
-    if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm3<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return; }
+    if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm3<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return (int)g.x; }
             MPD_GEMM3(128, 32, 4, 1)
             MPD_GEMM3(64, 32, 4, 1)
             MPD_GEMM3(32, 64, 2, 2)
@@ -1515,7 +1619,7 @@ void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
         }
     }
 #define MPD_GEMM2(TM, TN, WM, WN) \
-    if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm2<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return; }
+    if (t.tm == TM && t.tn == TN) { hipLaunchKernelGGL((conv_gemm2<MODE, TM, TN, WM, WN>), g, dim3(256), 0, s, a); return (int)g.x; }
     MPD_GEMM2(128, 128, 2, 2)
     MPD_GEMM2(128, 96, 2, 2)
     MPD_GEMM2(128, 64, 2, 2)
@@ -1536,6 +1640,7 @@ void launch_gemm2(const ConvArgs& a, int splits, hipStream_t s) {
     // Any other shape (every access is bounds-checked): the widest tile.
     const dim3 g2((unsigned)((a.M + 127) / 128), (unsigned)((a.N + 127) / 128), (unsigned)splits);
     hipLaunchKernelGGL((conv_gemm2<MODE, 128, 128, 2, 2>), g2, dim3(256), 0, s, a);
+    return (int)g2.x;
 }
 
 // The forward pass, z = conv(x, w) + bias, with bf16(silu(z)) written to
@@ -1616,6 +1721,33 @@ int mpd_nn_conv3x3_bwd_data(const uint16_t* dz, const uint16_t* w, float* dx, in
     const dim3 g((unsigned)((a.M + BM - 1) / BM), (unsigned)((a.N + BN - 1) / BN));
     if (vec_ok(Cin, Cout)) launch_gemm2<MODE_DGRAD>(a, 1, (hipStream_t)stream);
     else hipLaunchKernelGGL((conv_gemm<MODE_DGRAD, false>), g, dim3(256), 0, (hipStream_t)stream, a);
+    return finish(hipSuccess);
+}
+
+long long mpd_nn_bwd_data_silu_work_floats(int B, int H, int W, int Cin, int Cout) {
+    if (!conv_dims_ok(B, H, W, Cin, Cout)) return -1;
+    return ((long long)B * H * W + 15) / 16 * Cin;  // pixel tiles of at least 16 rows
+}
+
+int mpd_nn_conv3x3_bwd_data_silu(const uint16_t* dz, const uint16_t* w, const float* z_in, uint16_t* dz_in_bf16,
+                                 float* db_in, float* work, int B, int H, int W, int Cin, int Cout, void* stream) {
+    if (!dz || !w || !z_in || !dz_in_bf16 || !db_in || !work || !conv_dims_ok(B, H, W, Cin, Cout) ||
+        !vec_ok(Cin, Cout) || (Cin % 96 == 0 && Cin % 128 != 0))
+        return bad("conv3x3_bwd_data_silu: bad argument (channel counts multiples of 8; Cin not 96 x odd)");
+    ConvArgs a{};
+    a.dz = dz;
+    a.w = w;
+    a.zact = z_in;
+    a.dzo = dz_in_bf16;
+    a.bpart = work;
+    a.B = B, a.H = H, a.W = W, a.Cin = Cin, a.Cout = Cout;
+    a.M = B * H * W;
+    a.N = Cin;
+    a.K = 9 * Cout;
+    hipStream_t s = (hipStream_t)stream;
+    const int tiles = launch_gemm2<MODE_DGRAD>(a, 1, s);
+    hipLaunchKernelGGL(column_sum, dim3((unsigned)((Cin + 63) / 64)), dim3(1024), 0, s, (const float*)work, db_in,
+                       (int64_t)Cin, tiles);
     return finish(hipSuccess);
 }
 

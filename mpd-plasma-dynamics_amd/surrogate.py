@@ -54,6 +54,8 @@ def nn_lib():
         "mpd_nn_conv3x3_fwd": [_vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
         "mpd_nn_conv3x3_fwd_silu": [_vp, _vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
         "mpd_nn_conv3x3_bwd_data": [_vp, _vp, _vp, i, i, i, i, i, _vp],
+        "mpd_nn_conv3x3_bwd_data_silu": [_vp, _vp, _vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
+        "mpd_nn_bwd_data_silu_work_floats": [i, i, i, i, i],
         "mpd_nn_conv3x3_bwd_weight": [_vp, _vp, _vp, _vp, i, i, i, i, i, _vp],
         "mpd_nn_wgrad_work_floats": [i, i, i, i, i],
         "mpd_nn_bias_grad": [_vp, _vp, _vp, i, i, _vp],
@@ -97,6 +99,7 @@ def nn_lib():
 
 # Every symbol include/mpd_nn.h declares.
 NN_EXPORTS = ["mpd_nn_last_error", "mpd_nn_conv3x3_fwd", "mpd_nn_conv3x3_fwd_silu", "mpd_nn_conv3x3_bwd_data",
+              "mpd_nn_conv3x3_bwd_data_silu", "mpd_nn_bwd_data_silu_work_floats",
               "mpd_nn_wgrad_work_floats", "mpd_nn_silu_bwd_bias", "mpd_nn_upcat_fwd", "mpd_nn_upcat_bwd",
               "mpd_nn_pool_bwd_add",
               "mpd_nn_conv3x3_bwd_weight", "mpd_nn_bias_work_floats", "mpd_nn_bias_grad", "mpd_nn_silu_fwd", "mpd_nn_silu_bwd",
@@ -320,11 +323,15 @@ class UNet:
         bwork = max(L.mpd_nn_bias_work_floats(B * d["h"] * d["w"], d["co"]) for d in self.lay)
         self.bwork = DeviceArray(int(bwork))
         big = max(B * d["h"] * d["w"] * max(d["ci"], d["co"]) for d in self.lay)
-        # Gradient scratch: two ping-pong FP32 buffers, bf16 dz, and
-        # per-skip gradient buffers.
+        # Gradient scratch: two ping-pong FP32 buffers, two bf16 dz buffers
+        # and the fused data gradient's bias slices, and per-skip gradient
+        # buffers.
         self.g0 = DeviceArray(big)
         self.g1 = DeviceArray(big)
-        self.dzb = DeviceArray(big, "bf16")
+        self._dzb = [DeviceArray(big, "bf16"), DeviceArray(big, "bf16")]  # ping-pong: fused data gradients
+        self._dzi = 0
+        dwork = max(L.mpd_nn_bwd_data_silu_work_floats(B, d["h"], d["w"], d["ci"], d["co"]) for d in self.lay)
+        self.dwork = DeviceArray(int(dwork))
         self.dskip = [DeviceArray(B * H // 2 ** i * W // 2 ** i * ch[i])
                       for i in range(self.levels - 1)]
         self.loss_part = DeviceArray(1024)
@@ -372,51 +379,67 @@ class UNet:
         return d["z"]
 
     # -- backward ------------------------------------------------------------
-    def _conv_back(self, k, da, dx_out, B):
-        """Gradients of conv k from da = dL/d(output) (FP32 tensor); writes
-        dL/d(input) into dx_out (FP32, or None for the first conv)."""
+    def _conv_back(self, k, da, dx_out, B, fuse=False):
+        """Gradients of conv k.  da = dL/d(output) (FP32 tensor), or None when
+        this layer's dz and bias gradient were already written by the layer
+        above's data gradient (fused).  dx_out: dL/d(input) (FP32), or None
+        for the first conv; with fuse the data gradient instead writes conv
+        k - 1's dz (bf16) and bias gradient directly (its output feeds this
+        conv's input, with nothing in between)."""
         L, s = nn_lib(), _stream()
         d = self.lay[k]
-        z = _p(d["z"]) if d["act"] else None  # no activation: dz = da
-        _ck(L.mpd_nn_silu_bwd_bias(z, _p(da), _p(self.dzb), _p(self.gb[k]), _p(self.bwork), B * d["h"] * d["w"],
-                                   d["co"], s))
-        _ck(L.mpd_nn_conv3x3_bwd_weight(_p(d["a_in"]), _p(self.dzb), _p(self.gw[k]), _p(self.work), B, d["h"],
-                                        d["w"], d["ci"], d["co"], s))
-        if dx_out is not None:
-            _ck(L.mpd_nn_conv3x3_bwd_data(_p(self.dzb), _p(self.wb[k]), _p(dx_out), B, d["h"], d["w"], d["ci"],
+        dzb = self._dzb[self._dzi]
+        if da is not None:
+            z = _p(d["z"]) if d["act"] else None  # no activation: dz = da
+            _ck(L.mpd_nn_silu_bwd_bias(z, _p(da), _p(dzb), _p(self.gb[k]), _p(self.bwork), B * d["h"] * d["w"],
+                                       d["co"], s))
+        _ck(L.mpd_nn_conv3x3_bwd_weight(_p(d["a_in"]), _p(dzb), _p(self.gw[k]), _p(self.work), B, d["h"], d["w"],
+                                        d["ci"], d["co"], s))
+        if fuse:
+            below = self.lay[k - 1]
+            assert below["act"] and below["co"] == d["ci"]
+            self._dzi ^= 1
+            _ck(L.mpd_nn_conv3x3_bwd_data_silu(_p(dzb), _p(self.wb[k]), _p(below["z"]), _p(self._dzb[self._dzi]),
+                                               _p(self.gb[k - 1]), _p(self.dwork), B, d["h"], d["w"], d["ci"],
+                                               d["co"], s))
+        elif dx_out is not None:
+            _ck(L.mpd_nn_conv3x3_bwd_data(_p(dzb), _p(self.wb[k]), _p(dx_out), B, d["h"], d["w"], d["ci"],
                                           d["co"], s))
 
     def backward(self, dy, B):
-        """dy: dL/d(output) FP32 (B, H, W, cout).  Fills the gradients."""
+        """dy: dL/d(output) FP32 (B, H, W, cout).  Fills the gradients.  Where a
+        conv's input is the previous conv's output (the head and each level's
+        second conv), its data gradient also forms the previous conv's dz."""
         L, s = nn_lib(), _stream()
         k = len(self.lay) - 1
         cur, nxt = self.g0, self.g1
-        self._conv_back(k, dy, cur, B)
+        self._dzi = 0
+        self._conv_back(k, dy, None, B, fuse=True)  # head -> finest decoder's second conv
         k -= 1
         ch = [self.base * 2 ** i for i in range(self.levels)]
         for i in range(self.levels - 1):  # decoder levels, finest first
-            self._conv_back(k, cur, nxt, B)
-            cur, nxt = nxt, cur
+            self._conv_back(k, None if i == 0 else cur, None, B, fuse=True)
             k -= 1
             d = self.lay[k]
-            self._conv_back(k, cur, nxt, B)
+            self._conv_back(k, None, nxt, B)
             cur, nxt = nxt, cur
             k -= 1
             cu = d["ci"] - ch[i]
             _ck(L.mpd_nn_upcat_bwd(_p(cur), _p(nxt), cu, _p(self.dskip[i]), ch[i], B, d["h"] // 2, d["w"] // 2, s))
             cur, nxt = nxt, cur
-        for j in range(2):  # bottleneck
-            self._conv_back(k, cur, nxt, B)
-            cur, nxt = nxt, cur
-            k -= 1
+        # bottleneck
+        self._conv_back(k, cur, None, B, fuse=True)
+        k -= 1
+        self._conv_back(k, None, nxt, B)
+        cur, nxt = nxt, cur
+        k -= 1
         for i in range(self.levels - 2, -1, -1):  # encoder levels, coarsest first
             d = self.lay[k]
             _ck(L.mpd_nn_pool_bwd_add(_p(cur), _p(self.dskip[i]), _p(nxt), B, d["h"], d["w"], d["co"], s))
             cur, nxt = nxt, cur
-            self._conv_back(k, cur, nxt, B)
-            cur, nxt = nxt, cur
+            self._conv_back(k, cur, None, B, fuse=True)
             k -= 1
-            self._conv_back(k, cur, nxt if k > 0 else None, B)
+            self._conv_back(k, None, nxt if k > 0 else None, B)
             cur, nxt = nxt, cur
             k -= 1
         assert k == -1

@@ -199,8 +199,9 @@ def test_elementwise_kernels_match_oracle(nn):
 def test_fused_passes_equal_separate(nn, shape):
     """The fused entry points equal the separate passes bit for bit: the
     forward GEMM with SiLU in its epilogue against the forward then silu_fwd,
-    and the SiLU backward with the bias gradient (and without activation,
-    z = NULL) against silu_bwd / to_bf16 then bias_grad."""
+    the SiLU backward with the bias gradient (and without activation,
+    z = NULL) against silu_bwd / to_bf16 then bias_grad, and the data
+    gradient with the SiLU backward below it (bias in another order)."""
     B, H, W, Ci, Co = shape
     print(f"fused passes B={B} H={H} W={W} Cin={Ci} Cout={Co}", flush=True)
     rng = np.random.default_rng(sum(shape) + 1)
@@ -238,6 +239,31 @@ def test_fused_passes_equal_separate(nn, shape):
             ref = (da * S.silu_grad(_host(Z1))).reshape(P, Co)
             mag = np.abs(ref).sum(axis=0)
             assert np.all(np.abs(_host(DB2) - ref.sum(axis=0)) <= P * 2.0 ** -22 * mag + 1e-30)
+
+    # The data gradient with the SiLU backward of the layer below fused: this
+    # conv as that layer's consumer (its input's pre-activation z_in), against
+    # the data gradient then the SiLU backward with the bias: dz the same
+    # bits, the bias gradient within FP32 summation error (another order).
+    if Ci % 8 or Co % 8 or (Ci % 96 == 0 and Ci % 128):
+        return
+    zi = rng.normal(0, 2, size=(B, H, W, Ci)).astype(np.float32).astype(np.float64)
+    ZI = _dev_f32(zi)
+    dzl = S.bf16_round(rng.normal(size=(B, H, W, Co)))
+    DZL = _dev_bf16(dzl)
+    DX = _empty((B, H, W, Ci))
+    nn._ck(L.mpd_nn_conv3x3_bwd_data(nn._p(DZL), nn._p(Wt), nn._p(DX), B, H, W, Ci, Co, s))
+    n_in = B * H * W * Ci
+    DZ1, DZ2 = _empty((n_in,), bf16=True), _empty((n_in,), bf16=True)
+    DB1, DB2 = _empty((Ci,)), _empty((Ci,))
+    work_b = _empty((int(L.mpd_nn_bias_work_floats(P, Ci)),))
+    nn._ck(L.mpd_nn_silu_bwd_bias(nn._p(ZI), nn._p(DX), nn._p(DZ1), nn._p(DB1), nn._p(work_b), P, Ci, s))
+    work_f = _empty((int(L.mpd_nn_bwd_data_silu_work_floats(B, H, W, Ci, Co)),))
+    nn._ck(L.mpd_nn_conv3x3_bwd_data_silu(nn._p(DZL), nn._p(Wt), nn._p(ZI), nn._p(DZ2), nn._p(DB2), nn._p(work_f),
+                                          B, H, W, Ci, Co, s))
+    assert np.array_equal(DZ1.download(), DZ2.download())
+    d = (_host(DX) * S.silu_grad(zi)).reshape(P, Ci)
+    mag = np.abs(d).sum(axis=0)
+    assert np.all(np.abs(_host(DB2) - _host(DB1)) <= 2 * P * 2.0 ** -23 * mag + 1e-30)
 
 
 @pytest.mark.parametrize("B,H,W,Cu,Cs", [(2, 3, 5, 16, 8), (1, 25, 21, 64, 32), (2, 4, 6, 8, 24), (1, 50, 84, 64, 32)])

@@ -42,6 +42,9 @@ def test_fused_entry_points_reject_bad_arguments(nn):
     _fails(nn, nn.mpd_nn_conv3x3_fwd_silu(P, P, P, P, P, 0, 4, 4, 8, 8, s), "conv3x3_fwd_silu")
     _fails(nn, nn.mpd_nn_silu_bwd_bias(P, None, P, P, P, 10, 8, s), "silu_bwd_bias")
     _fails(nn, nn.mpd_nn_silu_bwd_bias(P, P, P, P, P, 0, 8, s), "silu_bwd_bias")
+    _fails(nn, nn.mpd_nn_conv3x3_bwd_data_silu(P, P, P, P, P, P, 1, 4, 4, 12, 8, s), "multiples of 8")
+    _fails(nn, nn.mpd_nn_conv3x3_bwd_data_silu(P, P, P, P, P, P, 1, 4, 4, 96, 8, s), "96")
+    _fails(nn, nn.mpd_nn_conv3x3_bwd_data_silu(P, P, None, P, P, P, 1, 4, 4, 8, 8, s), "conv3x3_bwd_data_silu")
     # Channel counts of the one-pass skip joins.
     _fails(nn, nn.mpd_nn_upcat_fwd(P, 12, P, 8, P, 1, 2, 2, s), "multiples of 8")
     _fails(nn, nn.mpd_nn_upcat_fwd(P, 8, P, 0, P, 1, 2, 2, s), "multiples of 8")
