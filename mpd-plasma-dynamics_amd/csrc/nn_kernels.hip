@@ -19,7 +19,9 @@
 // gradient, channel counts multiples of 16): row blocks staged in LDS and
 // read with gfx950's LDS transpose reads.  Fused passes for the training
 // step: the forward GEMM writes SiLU's output beside z, the SiLU backward
-// writes dz with the bias gradient's slices, the decoder's upsample +
+// writes dz with the bias gradient's slices (for a layer whose output feeds
+// the next conv directly, in that conv's data-gradient epilogue), the
+// decoder's upsample +
 // concatenation (and its backward) and the encoder's pooling backward +
 // skip gradient are one pass each, Adam runs over every tensor at once;
 // each equals its separate passes bit for bit.
