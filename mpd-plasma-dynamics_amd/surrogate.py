@@ -399,9 +399,21 @@ class UNet:
             below = self.lay[k - 1]
             assert below["act"] and below["co"] == d["ci"]
             self._dzi ^= 1
-            _ck(L.mpd_nn_conv3x3_bwd_data_silu(_p(dzb), _p(self.wb[k]), _p(below["z"]), _p(self._dzb[self._dzi]),
-                                               _p(self.gb[k - 1]), _p(self.dwork), B, d["h"], d["w"], d["ci"],
-                                               d["co"], s))
+            ci, co = d["ci"], d["co"]
+            if ci % 8 == 0 and co % 8 == 0 and not (ci % 96 == 0 and ci % 128):
+                _ck(L.mpd_nn_conv3x3_bwd_data_silu(_p(dzb), _p(self.wb[k]), _p(below["z"]),
+                                                   _p(self._dzb[self._dzi]), _p(self.gb[k - 1]), _p(self.dwork), B,
+                                                   d["h"], d["w"], ci, co, s))
+            else:
+                # Channel counts the fused entry does not take: the same
+                # results through dx and the separate SiLU backward.
+                if getattr(self, "_gtmp", None) is None or self._gtmp.numel() < self.g0.numel():
+                    self._gtmp = DeviceArray(self.g0.numel())
+                _ck(L.mpd_nn_conv3x3_bwd_data(_p(dzb), _p(self.wb[k]), _p(self._gtmp), B, d["h"], d["w"], ci, co,
+                                              s))
+                _ck(L.mpd_nn_silu_bwd_bias(_p(below["z"]), _p(self._gtmp), _p(self._dzb[self._dzi]),
+                                           _p(self.gb[k - 1]), _p(self.bwork), B * below["h"] * below["w"],
+                                           below["co"], s))
         elif dx_out is not None:
             _ck(L.mpd_nn_conv3x3_bwd_data(_p(dzb), _p(self.wb[k]), _p(dx_out), B, d["h"], d["w"], d["ci"],
                                           d["co"], s))
