@@ -1096,30 +1096,40 @@ __global__ void __launch_bounds__(256, 2) wgrad_lds(WgArgs a) {
 }
 
 // out[i] = sum over rows s < S of part[s][i], i < n: a block of 16 waves
-// per 64 outputs, wave w summing rows w, w + 16, ... into 8 running sums
-// (row w + 16 j + 128 k into sum j: eight independent loads in flight),
-// then its 8 sums and the 16 waves' results added in fixed tree orders
-// (deterministic, whatever the hardware schedule).
+// per 64 outputs.  With n < 64 (one block) a wave's lanes take CW = the
+// power of two >= n columns and 64 / CW interleaved row groups (lane l: column
+// l % CW, rows l / CW, + 64 / CW, ...); wave w's rows start at w 64 / CW.
+// Each lane keeps 8 running sums (eight independent loads in flight), added
+// in a fixed tree; the row groups are combined by fixed xor shuffles and the
+// 16 waves in a fixed tree order (deterministic, whatever the schedule).
 __global__ void __launch_bounds__(1024) column_sum(const float* part, float* out, int64_t n, int S) {
-    const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-    const int w = (int)(threadIdx.x >> 6);
+    int cw = 64;
+    if (n < 64) {
+        cw = 1;
+        while (cw < n) cw <<= 1;
+    }
+    const int R = 64 / cw;  // row groups per wave
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int64_t i = (int64_t)blockIdx.x * 64 + (lane % cw);
+    const int step = 16 * R;
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (i < n) {
-        int z = w;
-        for (; z + 112 < S; z += 128)
+        int z = w * R + lane / cw;
+        for (; z + 7 * step < S; z += 8 * step)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) acc[j] += part[(int64_t)(z + 16 * j) * n + i];
-        for (int j = 0; z < S; z += 16, ++j) acc[j] += part[(int64_t)z * n + i];
+            for (int j = 0; j < 8; ++j) acc[j] += part[(int64_t)(z + step * j) * n + i];
+        for (int j = 0; z < S; z += step, ++j) acc[j] += part[(int64_t)z * n + i];
     }
-    const float s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    float s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    for (int off = cw; off < 64; off <<= 1) s += __shfl_xor(s, off);
     __shared__ float sh[16][64];
-    sh[w][threadIdx.x & 63] = s;
+    sh[w][lane] = s;
     __syncthreads();
     for (int h = 8; h > 0; h >>= 1) {
-        if (w < h) sh[w][threadIdx.x & 63] += sh[w + h][threadIdx.x & 63];
+        if (w < h) sh[w][lane] += sh[w + h][lane];
         __syncthreads();
     }
-    if (w == 0 && i < n) out[i] = sh[0][threadIdx.x];
+    if (w == 0 && lane < cw && i < n) out[i] = sh[0][lane];
 }
 
 // Per-channel pixel sums (the bias gradient), pass 1: block (cb, s) sums
