@@ -105,6 +105,12 @@ int mpd_nn_upcat_bwd(const float* dy, float* dx, int Cu, float* dskip, int Cs, i
 /* dx = avgpool2 backward of dy + skip (H, W: the fine sizes, even; C a
  * multiple of 4), as mpd_nn_avgpool2_bwd then mpd_nn_add. */
 int mpd_nn_pool_bwd_add(const float* dy, const float* skip, float* dx, int B, int H, int W, int C, void* stream);
+/* mpd_nn_pool_bwd_add followed by mpd_nn_silu_bwd_bias (P = B H W, z the
+ * pre-activation [B][H][W][C]) in one pass, with no FP32 dx written: dz_bf16
+ * and db bit for bit those of the two calls.  H, W the fine sizes, even;
+ * B H W C below 2^31; `work` holds mpd_nn_bias_work_floats(B H W, C) floats. */
+int mpd_nn_pool_silu_bwd_bias(const float* dy, const float* skip, const float* z, uint16_t* dz_bf16, float* db,
+                              float* work, int B, int H, int W, int C, void* stream);
 
 /* Mean squared error of y against t over n values: dy = 2 (y - t) / n (dy
  * may be NULL) and mpd_nn_mse_blocks(n) partial sums of (y - t)^2 in part. */

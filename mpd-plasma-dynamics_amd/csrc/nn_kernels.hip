@@ -1139,7 +1139,7 @@ __global__ void __launch_bounds__(1024) column_sum(const float* part, float* out
 // wave reads 64 consecutive values per step; partial sum q (pixels p0 + q,
 // p0 + q + 4 PL, ...) of a channel sits at LDS slot q CW + channel and the
 // 4 PL partials are added in a fixed tree order (deterministic).  For flat
-// index i = p C + c, ld(i) loads the inputs and val(i, loaded) gives the
+// index i = p C + c, ld(i, p) loads the inputs and val(i, loaded) gives the
 // value summed (it may also write per-element outputs); four pixels' loads
 // are issued before their values are formed.
 __host__ __device__ inline int bias_cw(int C) { return C % 64 == 0 ? 64 : (64 % C == 0 ? C : 64); }
@@ -1160,7 +1160,7 @@ __device__ __forceinline__ void channel_slice_sum(float* part, int P, int C, int
         for (; p + 3 * step < p1; p += 4 * step) {
             const int64_t i0 = (int64_t)p * C + c, i1 = i0 + (int64_t)step * C, i2 = i1 + (int64_t)step * C,
                           i3 = i2 + (int64_t)step * C;
-            const auto t0 = ld(i0), t1 = ld(i1), t2 = ld(i2), t3 = ld(i3);
+            const auto t0 = ld(i0, p), t1 = ld(i1, p + step), t2 = ld(i2, p + 2 * step), t3 = ld(i3, p + 3 * step);
             s += val(i0, t0);
             s += val(i1, t1);
             s += val(i2, t2);
@@ -1168,7 +1168,7 @@ __device__ __forceinline__ void channel_slice_sum(float* part, int P, int C, int
         }
         for (; p < p1; p += step) {
             const int64_t i = (int64_t)p * C + c;
-            s += val(i, ld(i));
+            s += val(i, ld(i, p));
         }
     }
     __shared__ float sh[256];
@@ -1183,7 +1183,7 @@ __device__ __forceinline__ void channel_slice_sum(float* part, int P, int C, int
 
 __global__ void __launch_bounds__(256) bias_grad_part(const float* __restrict__ dz, float* __restrict__ part, int P,
                                                       int C, int rows) {
-    channel_slice_sum(part, P, C, rows, [&](int64_t i) { return dz[i]; }, [](int64_t, float v) { return v; });
+    channel_slice_sum(part, P, C, rows, [&](int64_t i, int) { return dz[i]; }, [](int64_t, float v) { return v; });
 }
 
 #define GRID_STRIDE(i, n) \
@@ -1210,9 +1210,36 @@ __global__ void __launch_bounds__(256) silu_bwd_bias_part(const float* __restric
                                                           uint16_t* __restrict__ dzb, float* __restrict__ part, int P,
                                                           int C, int rows) {
     channel_slice_sum(
-        part, P, C, rows, [&](int64_t i) { return float2{ACT ? z[i] : 0.f, da[i]}; },
+        part, P, C, rows, [&](int64_t i, int) { return float2{ACT ? z[i] : 0.f, da[i]}; },
         [&](int64_t i, float2 t) {
             const float d = ACT ? silu_grad(t.x, t.y) : t.y;
+            dzb[i] = f2bf(d);
+            return d;
+        });
+}
+
+// The encoder join's backward and the SiLU backward in one pass: da =
+// 0.25 dy[b][Y/2][X/2][c] + skip[b][Y][X][c] (pool_bwd_add's value, the
+// product rounded before the sum) is formed in registers and goes through
+// silu_bwd_bias_part's dz and bias slices, so the FP32 da never reaches
+// memory.  W is the fine width; B H W C < 2^31 and H, W even, so the pooled
+// row of fine row r = b H + Y is r / 2.  r = p / W is (p wm) >> wsh
+// (div_magic).
+__global__ void __launch_bounds__(256) pool_silu_bwd_bias_part(const float* __restrict__ dy,
+                                                               const float* __restrict__ skip,
+                                                               const float* __restrict__ z, uint16_t* __restrict__ dzb,
+                                                               float* __restrict__ part, int P, int C, int W,
+                                                               uint64_t wm, int wsh, int rows) {
+    channel_slice_sum(
+        part, P, C, rows,
+        [&](int64_t i, int p) {
+            const int c = (int)i - p * C, r = (int)(((uint64_t)(uint32_t)p * wm) >> wsh), X = p - r * W;
+            const int j = ((r >> 1) * (W >> 1) + (X >> 1)) * C + c;
+            return float3{z[i], dy[j], skip[i]};
+        },
+        [&](int64_t i, float3 t) {
+#pragma clang fp contract(off)
+            const float d = silu_grad(t.x, 0.25f * t.y + t.z);
             dzb[i] = f2bf(d);
             return d;
         });
@@ -1676,6 +1703,21 @@ void launch_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, fl
 
 }  // namespace
 
+// floor(n / d) = (n m) >> sh for 0 <= n < 2^31 and d >= 1, with l =
+// ceil(log2 d), sh = 31 + l and m = ceil(2^sh / d) <= 2^32, so n m < 2^63
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, theorem 4.2).
+struct DivMagic {
+    uint64_t m;
+    int sh;
+};
+static DivMagic div_magic(int d) {
+    int l = 0;
+    while ((1LL << l) < d) ++l;
+    const int sh = 31 + l;
+    return {((1ULL << sh) + (uint64_t)d - 1) / (uint64_t)d, sh};
+}
+
 extern "C" {
 
 const char* mpd_nn_last_error(void) { return g_nn_error.c_str(); }
@@ -1925,6 +1967,22 @@ int mpd_nn_pool_bwd_add(const float* dy, const float* skip, float* dx, int B, in
         return bad("pool_bwd_add: bad argument (H, W even, C a positive multiple of 4)");
     hipLaunchKernelGGL(pool_bwd_add, dim3(blocks_for(runs)), dim3(256), 0, (hipStream_t)stream, dy, skip, dx, B, H, W,
                        C);
+    return finish(hipSuccess);
+}
+
+int mpd_nn_pool_silu_bwd_bias(const float* dy, const float* skip, const float* z, uint16_t* dz_bf16, float* db,
+                              float* work, int B, int H, int W, int C, void* stream) {
+    if (!dy || !skip || !z || !dz_bf16 || !db || !work || B <= 0 || H <= 0 || W <= 0 || H % 2 || W % 2 || C <= 0 ||
+        1LL * B * H * W * C >= (1LL << 31))
+        return bad("pool_silu_bwd_bias: bad argument (H, W even, B H W C below 2^31)");
+    const int P = B * H * W;
+    const int S = bias_slices(P, C);
+    const int rows = (P + S - 1) / S;
+    hipStream_t s = (hipStream_t)stream;
+    const DivMagic wd = div_magic(W);
+    hipLaunchKernelGGL(pool_silu_bwd_bias_part, dim3((unsigned)((C + bias_cw(C) - 1) / bias_cw(C)), (unsigned)S),
+                       dim3(256), 0, s, dy, skip, z, dz_bf16, work, P, C, W, wd.m, wd.sh, rows);
+    hipLaunchKernelGGL(column_sum, dim3((unsigned)((C + 63) / 64)), dim3(1024), 0, s, (const float*)work, db, (int64_t)C, S);
     return finish(hipSuccess);
 }
 

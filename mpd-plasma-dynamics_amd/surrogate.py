@@ -74,6 +74,7 @@ def nn_lib():
         "mpd_nn_upcat_fwd": [_vp, i, _vp, i, _vp, i, i, i, _vp],
         "mpd_nn_upcat_bwd": [_vp, _vp, i, _vp, i, i, i, i, _vp],
         "mpd_nn_pool_bwd_add": [_vp, _vp, _vp, i, i, i, i, _vp],
+        "mpd_nn_pool_silu_bwd_bias": [_vp, _vp, _vp, _vp, _vp, _vp, i, i, i, i, _vp],
         "mpd_nn_mse_blocks": [ll],
         "mpd_nn_mse": [_vp, _vp, _vp, _vp, ll, _vp],
         "mpd_nn_adam": [_vp, _vp, _vp, _vp, _vp, ll, f, f, f, f, i, f, _vp],
@@ -101,7 +102,7 @@ def nn_lib():
 NN_EXPORTS = ["mpd_nn_last_error", "mpd_nn_conv3x3_fwd", "mpd_nn_conv3x3_fwd_silu", "mpd_nn_conv3x3_bwd_data",
               "mpd_nn_conv3x3_bwd_data_silu", "mpd_nn_bwd_data_silu_work_floats",
               "mpd_nn_wgrad_work_floats", "mpd_nn_silu_bwd_bias", "mpd_nn_upcat_fwd", "mpd_nn_upcat_bwd",
-              "mpd_nn_pool_bwd_add",
+              "mpd_nn_pool_bwd_add", "mpd_nn_pool_silu_bwd_bias",
               "mpd_nn_conv3x3_bwd_weight", "mpd_nn_bias_work_floats", "mpd_nn_bias_grad", "mpd_nn_silu_fwd", "mpd_nn_silu_bwd",
               "mpd_nn_to_bf16", "mpd_nn_avgpool2_fwd", "mpd_nn_avgpool2_bwd", "mpd_nn_upsample2_fwd",
               "mpd_nn_upsample2_bwd", "mpd_nn_concat_fwd", "mpd_nn_concat_bwd", "mpd_nn_add", "mpd_nn_mse_blocks",
@@ -447,9 +448,12 @@ class UNet:
         k -= 1
         for i in range(self.levels - 2, -1, -1):  # encoder levels, coarsest first
             d = self.lay[k]
-            _ck(L.mpd_nn_pool_bwd_add(_p(cur), _p(self.dskip[i]), _p(nxt), B, d["h"], d["w"], d["co"], s))
-            cur, nxt = nxt, cur
-            self._conv_back(k, cur, None, B, fuse=True)
+            assert d["act"]
+            # Pooling backward + skip gradient + SiLU backward + bias slices
+            # in one pass: this conv's dz, with no FP32 gradient in between.
+            _ck(L.mpd_nn_pool_silu_bwd_bias(_p(cur), _p(self.dskip[i]), _p(d["z"]), _p(self._dzb[self._dzi]),
+                                            _p(self.gb[k]), _p(self.bwork), B, d["h"], d["w"], d["co"], s))
+            self._conv_back(k, None, None, B, fuse=True)
             k -= 1
             self._conv_back(k, None, nxt if k > 0 else None, B)
             cur, nxt = nxt, cur

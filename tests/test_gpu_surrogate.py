@@ -306,6 +306,20 @@ def test_fused_joins_equal_separate(nn, B, H, W, Cu, Cs):
     assert np.array_equal(PB1.download(), PB2.download())
     np.testing.assert_allclose(_host(PB2), S.pool_bwd(g.astype(np.float64)) + dy[..., Cu:], rtol=1e-6, atol=1e-7)
 
+    # ... and with the SiLU backward and bias slices in the same pass.
+    z = rng.normal(size=(B, Hf, Wf, Cs)).astype(np.float32)
+    Z = _dev_f32(z)
+    DZ1, DZ2 = _empty((P * Cs,), bf16=True), _empty((P * Cs,), bf16=True)
+    DB1, DB2 = _empty((Cs,)), _empty((Cs,))
+    work = _empty((int(L.mpd_nn_bias_work_floats(P, Cs)),))
+    nn._ck(L.mpd_nn_silu_bwd_bias(nn._p(Z), nn._p(PB2), nn._p(DZ1), nn._p(DB1), nn._p(work), P, Cs, s))
+    nn._ck(L.mpd_nn_pool_silu_bwd_bias(nn._p(G), nn._p(DS2), nn._p(Z), nn._p(DZ2), nn._p(DB2), nn._p(work), B, Hf,
+                                       Wf, Cs, s))
+    assert np.array_equal(DZ1.download(), DZ2.download())
+    assert np.array_equal(DB1.download(), DB2.download())
+    d = (_host(PB2) * S.silu_grad(z)).reshape(P, Cs)
+    assert np.all(np.abs(_host(DB2) - d.sum(axis=0)) <= 4 * P * 2.0 ** -23 * np.abs(d).sum(axis=0) + 1e-30)
+
 
 def test_adam_multi_equals_separate(nn):
     """One multi-tensor Adam launch (35 tensors: two launches of the table,

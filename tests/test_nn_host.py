@@ -52,6 +52,9 @@ def test_fused_entry_points_reject_bad_arguments(nn):
     _fails(nn, nn.mpd_nn_upcat_bwd(P, P, 8, None, 8, 1, 2, 2, s), "upcat_bwd")
     _fails(nn, nn.mpd_nn_pool_bwd_add(P, P, P, 1, 3, 4, 8, s), "H, W even")
     _fails(nn, nn.mpd_nn_pool_bwd_add(P, P, P, 1, 4, 4, 6, s), "multiple of 4")
+    _fails(nn, nn.mpd_nn_pool_silu_bwd_bias(P, P, P, P, P, P, 1, 4, 3, 8, s), "H, W even")
+    _fails(nn, nn.mpd_nn_pool_silu_bwd_bias(P, P, None, P, P, P, 1, 4, 4, 8, s), "pool_silu_bwd_bias")
+    _fails(nn, nn.mpd_nn_pool_silu_bwd_bias(P, P, P, P, P, P, 64, 512, 512, 256, s), "2^31")
     # 32-bit index limit of the one-pass joins.
     _fails(nn, nn.mpd_nn_upcat_fwd(P, 512, P, 512, P, 64, 512, 512, s), "upcat_fwd")
 
